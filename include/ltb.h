@@ -1,0 +1,195 @@
+/*
+ * ltb.h -- C ABI of the MI355X density-matrix path (KPE / SP2 / SCC).
+ *
+ * Plain C types only.  Host arrays are caller-owned and copied; device
+ * buffers belong to an ltb_ctx.  One ltb_ctx = one HIP stream; calls on one
+ * context are not thread-safe (the reference's entry points are collective
+ * over PETSC_COMM_WORLD and run in lockstep, which this mirrors).  Every
+ * function returns an ltb_status; ltb_last_error() gives the message.
+ *
+ * Each entry point names the reference function it replaces
+ * (pghale/largeTB, withSP2_deltaH.cpp unless stated).
+ */
+#ifndef LTB_H
+#define LTB_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum ltb_status {
+  LTB_OK = 0,
+  LTB_ERR_ARG = 1,        /* bad argument / size                         */
+  LTB_ERR_GEOMETRY = 2,   /* Si-O closer than 0.01 Bohr, unknown pair (:219-222, :338-341) */
+  LTB_ERR_NEIGHBORS = 3,  /* more than maxN_A = 20 neighbours (:357-359) */
+  LTB_ERR_NONFINITE = 4,  /* NaN/Inf met in H or a reduction           */
+  LTB_NOT_CONVERGED = 5,  /* SCC hit its iteration cap (:1820)           */
+  LTB_ERR_DEVICE = 6,     /* HIP runtime error                           */
+  LTB_ERR_STATE = 7,      /* call order (e.g. no system loaded)          */
+  LTB_ERR_NOMEM = 8
+};
+
+enum ltb_lattice {
+  LTB_LATTICE_CONFIG = 0,    /* a=4.913 A, c=5.405 A, 3-D periodic           */
+  LTB_LATTICE_REFERENCE = 1  /* a=9.725, c=10.7 Bohr, x/y images (:380,:400) */
+};
+
+enum ltb_precision {
+  LTB_FP64 = 0,       /* recursion vectors and H~ in FP64                    */
+  LTB_FP32_ACC64 = 1  /* FP32 vectors and H~, moments accumulated in FP64    */
+};
+
+typedef struct ltb_system ltb_system; /* host: geometry, neighbours, H0 (BSR 4x4), Gamma */
+typedef struct ltb_ctx ltb_ctx;       /* device context                                    */
+
+typedef struct {
+  int L, M, N;             /* cells along R1, R2, R3                           */
+  int lattice;             /* enum ltb_lattice                                  */
+  double sigma_ang;        /* Gaussian displacement per Cartesian component (A) */
+  unsigned long long seed; /* SplitMix64 seed of the displacements              */
+} ltb_quartz_desc;
+
+/* ---- host side: structure, neighbour list, H0 and Gamma assembly ------------ */
+
+/* arrangeCoordinates + initializeCoordinates + getHaloAtoms + constructH +
+ * initializeForSCF (:669-738, :504-667, :372-501, :969-1066, :854-967). */
+int ltb_system_quartz(const ltb_quartz_desc* desc, ltb_system** out);
+
+/* Same assembly from explicit coordinates (Bohr); Z is 14 (Si) or 8 (O).
+ * cell9 rows are the periodic vectors; periodic_mask bit d enables images
+ * along row d. */
+int ltb_system_create(int natoms, const int* Z, const double* xyz_bohr, const double* cell9,
+                      int periodic_mask, ltb_system** out);
+void ltb_system_destroy(ltb_system* sys);
+const char* ltb_system_last_error(void); /* thread-local message of the last failure */
+
+/* sizes: atoms, BSR blocks of H0, nonzeros of Gamma, valence electrons */
+int ltb_system_sizes(const ltb_system* sys, int* natoms, int* nnzb, int* gamma_nnz,
+                     double* nelectrons);
+int ltb_system_coords(const ltb_system* sys, int* Z, double* xyz_bohr);
+/* H0 as BSR 4x4, row-major blocks; per row the diagonal block first, then
+ * neighbour blocks by ascending column (MatSetValues targets, :993, :1022). */
+int ltb_system_hamiltonian(const ltb_system* sys, int* rowptr, int* col, double* val16);
+/* Gamma as CSR with the same row order (Gamma_ab, :887-896). */
+int ltb_system_gamma(const ltb_system* sys, int* rowptr, int* col, double* val);
+/* neutral per-spin populations (Si 2, O 3; arrangeCoordinates :714-720 with
+ * the O branch the reference lacks). */
+int ltb_system_neutral(const ltb_system* sys, double* p0);
+
+/* ---- device context ---------------------------------------------------------- */
+int ltb_create(int device, ltb_ctx** out);
+void ltb_destroy(ltb_ctx* ctx);
+const char* ltb_last_error(const ltb_ctx* ctx);
+
+/* Upload H0, Gamma and the orbital->atom map (initializeForH + MatAssembly). */
+int ltb_load_system(ltb_ctx* ctx, const ltb_system* sys);
+
+/* constructdeltaH (:1414-1487): v = Gamma (p - p0) on device (FP64), H_total =
+ * H0 + diag(v repeated over the 4 orbitals).  p: per-spin populations [natoms]. */
+int ltb_set_populations(ltb_ctx* ctx, const double* p);
+int ltb_get_shift(ltb_ctx* ctx, double* v_atom);
+
+/* GershGorinParallel (:144-209) on H_total, FP64 device reduction. */
+int ltb_gershgorin(ltb_ctx* ctx, double* emin, double* emax);
+
+/* runKPM rescale (:1554-1560): H~ = (H_total - b)/a, stored on device in
+ * FP64 and FP32. */
+int ltb_rescale(ltb_ctx* ctx, double a, double b);
+
+/* Exact-trace Chebyshev moments of H~ (getKPE recursion :1139-1168 over all
+ * unit vectors, as in the commented exact mode :1361-1371):
+ *   atom_moments[m*natoms + A] = sum_{j in A} [T_m(H~)]_jj,  m = 0..M
+ *   total_moments[m]           = Tr T_m(H~)
+ * Either output may be NULL (results stay on device for ltb_density). */
+int ltb_kpe_moments(ltb_ctx* ctx, int M, int precision, double* atom_moments,
+                    double* total_moments);
+
+typedef struct {
+  int M;          /* moments used by P (m < M)                               */
+  double kT;      /* Hartree; <= 0 selects the reference step function      */
+  int K;          /* quadrature nodes for the Fermi-Dirac coefficients       */
+  double Ne;      /* electrons, both spins                                   */
+} ltb_fermi_params;
+
+typedef struct {
+  double mu;      /* scaled chemical potential (H~ frame)                    */
+  double mu_ha;   /* Hartree                                                 */
+  double a, b;    /* rescale used                                            */
+  double nelec;   /* electrons reproduced by the bisected mu                 */
+  double eband;   /* 2 Tr(P H_total), Hartree                                */
+} ltb_density_result;
+
+/* computeDensityMatrix (:1401-1411) in exact mode: mu bisected on device in
+ * FP64 to Ne (replaces runKPM's DOS search :1663-1685), Fermi-Dirac (or step)
+ * coefficients times the gmMs kernel (:1100-1132), per-atom populations
+ * (computeDiagonalElements + orbital2Atom, :1296-1399) and band energy, from
+ * the moments of the last ltb_kpe_moments call.  pop: per-spin [natoms]. */
+int ltb_density(ltb_ctx* ctx, const ltb_fermi_params* fp, double* pop, ltb_density_result* res);
+
+typedef struct {
+  int M;              /* Chebyshev moments                                       */
+  double kT;          /* Hartree (config: 0.05 eV)                               */
+  int K;              /* FD quadrature nodes                                     */
+  double mix;         /* linear charge mixing (config 0.2)                      */
+  double tol;         /* |q_out - q_in|_inf in e (config 1e-6)                  */
+  int max_iter;       /* cap (reference: 20, :1820)                             */
+  int precision;      /* enum ltb_precision for the bulk iterations            */
+  double switch_tol;  /* FP32 bulk -> FP64 once |dq| < switch_tol (0: never)   */
+  int use_graph;      /* capture the device part of an iteration in a hipGraph */
+} ltb_scc_params;
+
+typedef struct {
+  int iterations;
+  int fp64_iterations;
+  double dq;          /* last |q_out - q_in|_inf (e)                            */
+  double mu_ha;
+  double eband;       /* Hartree, at the last iteration                          */
+  double seconds;     /* wall time of the loop                                   */
+} ltb_scc_result;
+
+/* SCC driver (main :1815-1840): loop of constructdeltaH, runKPM (bounds +
+ * rescale), computeDensityMatrix and linear mixing of the populations (in
+ * place of updateHtotal's Hamiltonian mixing, :1489-1526).  p_guess (per
+ * spin) may be NULL (neutral start).  p_out receives the final q_out / 2.
+ * hist (optional, [max_iter]) receives |dq| per iteration. */
+int ltb_scc_run(ltb_ctx* ctx, const ltb_scc_params* prm, const double* p_guess, double* p_out,
+                double* hist, ltb_scc_result* res);
+
+/* getKPE (:1068-1185) on a block of vectors: Y = sum_m w[m] T_m(H~) X, with
+ * w = (c_0/2, c_1 g_1, ...).  X, Y: [ncols][norb] column vectors (host). */
+int ltb_kpe_apply(ltb_ctx* ctx, int M, const double* w, int ncols, const double* X, double* Y,
+                  int precision);
+
+/* SP2 (getSP2 :1187-1215 / computeAlphas :1217-1294) on an explicit dense
+ * matrix: X0 = (emax I - H_total)/(emax - emin) from the Gershgorin bounds,
+ * X <- X^2 if 2 tr X > Ne else 2X - X^2, FP32 MFMA GEMMs until the
+ * idempotency error stops falling (or max_iter), then fp64_tail FP64 MFMA
+ * iterations.  diagP (optional, [norb]) and pop (optional, per-spin [natoms])
+ * receive diag(X) and its per-atom sums. */
+typedef struct {
+  int iterations, fp64_iterations;
+  double trace;   /* tr X at the end (per spin)                  */
+  double idem;    /* |tr X - tr X^2| at the end                  */
+  double gemm_seconds; /* time inside the X^2 products          */
+  double gemm_flops;   /* 2 n^3 per product, summed             */
+} ltb_sp2_result;
+int ltb_sp2_dense(ltb_ctx* ctx, double Ne, int max_iter, int fp64_tail, double* diagP, double* pop,
+                  ltb_sp2_result* res);
+
+/* Timing of the last ltb_kpe_moments call: number of recursion-step launches,
+ * their summed device time (HIP events on the context stream), and the
+ * algorithmic bytes they move (3 vector streams + H~ blocks + row pointers). */
+int ltb_kpe_stats(ltb_ctx* ctx, int* launches, double* step_seconds, double* alg_bytes);
+
+/* Record HIP events around every recursion-step launch (on the context
+ * stream) so ltb_kpe_stats reports their summed device time. */
+int ltb_set_kpe_timing(ltb_ctx* ctx, int on);
+
+/* Tuning knobs of the Chebyshev kernel (0 = keep current / autotuned):
+ * columns in flight per chunk and atoms per wave. */
+int ltb_set_kpe_tiling(ltb_ctx* ctx, int chunk_cols, int atoms_per_wave);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

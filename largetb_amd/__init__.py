@@ -1,0 +1,311 @@
+"""largetb_amd: MI355X-native density-matrix path (KPE / SP2 / SCC) of
+pghale/largeTB behind a C ABI (include/ltb.h).
+
+This module is a ctypes front end of the in-tree ``libltb.so``.  It raises if
+the library is missing: there is no CPU fallback for the device calls.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libltb.so")
+
+LTB_OK = 0
+LTB_ERR_ARG = 1
+LTB_ERR_GEOMETRY = 2
+LTB_ERR_NEIGHBORS = 3
+LTB_ERR_NONFINITE = 4
+LTB_NOT_CONVERGED = 5
+LTB_ERR_DEVICE = 6
+LTB_ERR_STATE = 7
+LTB_ERR_NOMEM = 8
+LATTICE_CONFIG = 0
+LATTICE_REFERENCE = 1
+FP64 = 0
+FP32_ACC64 = 1
+HARTREE_EV = 27.211386
+
+_lib = None
+
+
+class LtbError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"status {status}: {msg}")
+        self.status = status
+
+
+class QuartzDesc(C.Structure):
+    _fields_ = [("L", C.c_int), ("M", C.c_int), ("N", C.c_int), ("lattice", C.c_int),
+                ("sigma_ang", C.c_double), ("seed", C.c_ulonglong)]
+
+
+class FermiParams(C.Structure):
+    _fields_ = [("M", C.c_int), ("kT", C.c_double), ("K", C.c_int), ("Ne", C.c_double)]
+
+
+class DensityResult(C.Structure):
+    _fields_ = [("mu", C.c_double), ("mu_ha", C.c_double), ("a", C.c_double), ("b", C.c_double),
+                ("nelec", C.c_double), ("eband", C.c_double)]
+
+
+class SccParams(C.Structure):
+    _fields_ = [("M", C.c_int), ("kT", C.c_double), ("K", C.c_int), ("mix", C.c_double),
+                ("tol", C.c_double), ("max_iter", C.c_int), ("precision", C.c_int),
+                ("switch_tol", C.c_double), ("use_graph", C.c_int)]
+
+
+class SccResult(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("fp64_iterations", C.c_int), ("dq", C.c_double),
+                ("mu_ha", C.c_double), ("eband", C.c_double), ("seconds", C.c_double)]
+
+
+class Sp2Result(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("fp64_iterations", C.c_int), ("trace", C.c_double),
+                ("idem", C.c_double), ("gemm_seconds", C.c_double), ("gemm_flops", C.c_double)]
+
+
+PI = C.POINTER(C.c_int)
+PD = C.POINTER(C.c_double)
+VP = C.c_void_p
+
+# every symbol declared in include/ltb.h with its ctypes signature
+SIGNATURES = {
+    "ltb_system_quartz": (C.c_int, [C.POINTER(QuartzDesc), C.POINTER(VP)]),
+    "ltb_system_create": (C.c_int, [C.c_int, PI, PD, PD, C.c_int, C.POINTER(VP)]),
+    "ltb_system_destroy": (None, [VP]),
+    "ltb_system_last_error": (C.c_char_p, []),
+    "ltb_system_sizes": (C.c_int, [VP, PI, PI, PI, PD]),
+    "ltb_system_coords": (C.c_int, [VP, PI, PD]),
+    "ltb_system_hamiltonian": (C.c_int, [VP, PI, PI, PD]),
+    "ltb_system_gamma": (C.c_int, [VP, PI, PI, PD]),
+    "ltb_system_neutral": (C.c_int, [VP, PD]),
+    "ltb_create": (C.c_int, [C.c_int, C.POINTER(VP)]),
+    "ltb_destroy": (None, [VP]),
+    "ltb_last_error": (C.c_char_p, [VP]),
+    "ltb_load_system": (C.c_int, [VP, VP]),
+    "ltb_set_populations": (C.c_int, [VP, PD]),
+    "ltb_get_shift": (C.c_int, [VP, PD]),
+    "ltb_gershgorin": (C.c_int, [VP, PD, PD]),
+    "ltb_rescale": (C.c_int, [VP, C.c_double, C.c_double]),
+    "ltb_kpe_moments": (C.c_int, [VP, C.c_int, C.c_int, PD, PD]),
+    "ltb_density": (C.c_int, [VP, C.POINTER(FermiParams), PD, C.POINTER(DensityResult)]),
+    "ltb_scc_run": (C.c_int, [VP, C.POINTER(SccParams), PD, PD, PD, C.POINTER(SccResult)]),
+    "ltb_kpe_apply": (C.c_int, [VP, C.c_int, PD, C.c_int, PD, PD, C.c_int]),
+    "ltb_sp2_dense": (C.c_int, [VP, C.c_double, C.c_int, C.c_int, PD, PD, C.POINTER(Sp2Result)]),
+    "ltb_kpe_stats": (C.c_int, [VP, PI, PD, PD]),
+    "ltb_set_kpe_timing": (C.c_int, [VP, C.c_int]),
+    "ltb_set_kpe_tiling": (C.c_int, [VP, C.c_int, C.c_int]),
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with python -c 'import "
+                              "__graft_entry__ as g; g.build()' (no CPU fallback exists)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(PD)
+
+
+def _i(a):
+    return a.ctypes.data_as(PI)
+
+
+def _sys_check(st):
+    if st != LTB_OK:
+        raise LtbError(st, lib().ltb_system_last_error().decode())
+
+
+class System:
+    """Host-side structure: geometry, neighbour list, H0 (BSR 4x4) and Gamma."""
+
+    def __init__(self, handle):
+        self._h = handle
+        nat, nnzb, gnnz, ne = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        _sys_check(lib().ltb_system_sizes(handle, C.byref(nat), C.byref(nnzb), C.byref(gnnz),
+                                          C.byref(ne)))
+        self.nat, self.nnzb, self.gnnz, self.nelectrons = nat.value, nnzb.value, gnnz.value, ne.value
+        self.norb = 4 * self.nat
+
+    @classmethod
+    def quartz(cls, L, M, N, lattice=LATTICE_CONFIG, sigma_ang=0.0, seed=1):
+        d = QuartzDesc(L, M, N, lattice, sigma_ang, seed)
+        h = VP()
+        _sys_check(lib().ltb_system_quartz(C.byref(d), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_coords(cls, Z, xyz, cell=None, periodic_mask=7):
+        Z = np.ascontiguousarray(Z, np.int32)
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+        cp = None
+        if cell is not None:
+            cell = np.ascontiguousarray(cell, np.float64).reshape(9)
+            cp = _d(cell)
+        h = VP()
+        _sys_check(lib().ltb_system_create(len(Z), _i(Z), _d(xyz), cp, periodic_mask, C.byref(h)))
+        return cls(h)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.ltb_system_destroy(self._h)
+            self._h = None
+
+    def coords(self):
+        Z = np.zeros(self.nat, np.int32)
+        xyz = np.zeros(3 * self.nat)
+        _sys_check(lib().ltb_system_coords(self._h, _i(Z), _d(xyz)))
+        return Z, xyz.reshape(-1, 3)
+
+    def hamiltonian(self):
+        rp = np.zeros(self.nat + 1, np.int32)
+        col = np.zeros(self.nnzb, np.int32)
+        val = np.zeros(self.nnzb * 16)
+        _sys_check(lib().ltb_system_hamiltonian(self._h, _i(rp), _i(col), _d(val)))
+        return rp, col, val.reshape(-1, 4, 4)
+
+    def gamma(self):
+        rp = np.zeros(self.nat + 1, np.int32)
+        col = np.zeros(self.gnnz, np.int32)
+        val = np.zeros(self.gnnz)
+        _sys_check(lib().ltb_system_gamma(self._h, _i(rp), _i(col), _d(val)))
+        return rp, col, val
+
+    def neutral(self):
+        p0 = np.zeros(self.nat)
+        _sys_check(lib().ltb_system_neutral(self._h, _d(p0)))
+        return p0
+
+    def dense_h(self):
+        rp, col, val = self.hamiltonian()
+        H = np.zeros((self.norb, self.norb))
+        for i in range(self.nat):
+            for k in range(rp[i], rp[i + 1]):
+                H[4 * i:4 * i + 4, 4 * col[k]:4 * col[k] + 4] = val[k]
+        return H
+
+
+class Context:
+    """Device context on one GPU (one HIP stream)."""
+
+    def __init__(self, device=0):
+        h = VP()
+        st = lib().ltb_create(device, C.byref(h))
+        if st != LTB_OK:
+            raise LtbError(st, lib().ltb_last_error(None).decode())
+        self._h = h
+        self.system = None
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            lib().ltb_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, st):
+        if st != LTB_OK:
+            raise LtbError(st, lib().ltb_last_error(self._h).decode())
+
+    def load(self, system):
+        self._check(lib().ltb_load_system(self._h, system._h))
+        self.system = system
+        self.nat = system.nat
+        self.norb = system.norb
+        return self
+
+    def set_populations(self, p=None):
+        pp = _d(np.ascontiguousarray(p, np.float64)) if p is not None else None
+        self._check(lib().ltb_set_populations(self._h, pp))
+
+    def shift(self):
+        v = np.zeros(self.nat)
+        self._check(lib().ltb_get_shift(self._h, _d(v)))
+        return v
+
+    def gershgorin(self):
+        lo, hi = C.c_double(), C.c_double()
+        self._check(lib().ltb_gershgorin(self._h, C.byref(lo), C.byref(hi)))
+        return lo.value, hi.value
+
+    def rescale(self, a, b):
+        self._check(lib().ltb_rescale(self._h, a, b))
+
+    def kpe_moments(self, M, precision=FP64, want_atoms=True):
+        tot = np.zeros(M + 1)
+        am = np.zeros((M + 1) * self.nat) if want_atoms else None
+        self._check(lib().ltb_kpe_moments(self._h, M, precision, _d(am) if am is not None else None,
+                                          _d(tot)))
+        return (am.reshape(M + 1, self.nat) if am is not None else None), tot
+
+    def density(self, M, kT_ev=0.05, K=8192, Ne=None):
+        kT = kT_ev / HARTREE_EV if kT_ev > 0 else 0.0
+        fp = FermiParams(M, kT, K, self.system.nelectrons if Ne is None else Ne)
+        pop = np.zeros(self.nat)
+        r = DensityResult()
+        self._check(lib().ltb_density(self._h, C.byref(fp), _d(pop), C.byref(r)))
+        return dict(pop=pop, mu=r.mu, mu_ha=r.mu_ha, a=r.a, b=r.b, nelec=r.nelec, eband=r.eband)
+
+    def scc(self, M, guess=None, kT_ev=0.05, K=8192, mix=0.2, tol=1e-6, max_iter=200,
+            precision=FP64, switch_tol=0.0, use_graph=0, check=True):
+        kT = kT_ev / HARTREE_EV if kT_ev > 0 else 0.0
+        prm = SccParams(M, kT, K, mix, tol, max_iter, precision, switch_tol, use_graph)
+        pout = np.zeros(self.nat)
+        hist = np.zeros(max_iter)
+        r = SccResult()
+        g = _d(np.ascontiguousarray(guess, np.float64)) if guess is not None else None
+        st = lib().ltb_scc_run(self._h, C.byref(prm), g, _d(pout), _d(hist), C.byref(r))
+        if check and st not in (LTB_OK,):
+            self._check(st)
+        return dict(status=st, pop=pout, q=2 * pout, iters=r.iterations, fp64_iters=r.fp64_iterations,
+                    dq=r.dq, mu_ha=r.mu_ha, eband=r.eband, seconds=r.seconds,
+                    hist=hist[:r.iterations])
+
+    def kpe_apply(self, M, w, X, precision=FP64):
+        X = np.ascontiguousarray(X, np.float64)
+        if X.ndim == 1:
+            X = X[None, :]
+        w = np.ascontiguousarray(w, np.float64)
+        Y = np.zeros_like(X)
+        self._check(lib().ltb_kpe_apply(self._h, M, _d(w), X.shape[0], _d(X), _d(Y), precision))
+        return Y
+
+    def sp2_dense(self, Ne=None, max_iter=100, fp64_tail=2):
+        diag = np.zeros(self.norb)
+        pop = np.zeros(self.nat)
+        r = Sp2Result()
+        self._check(lib().ltb_sp2_dense(self._h, self.system.nelectrons if Ne is None else Ne,
+                                        max_iter, fp64_tail, _d(diag), _d(pop), C.byref(r)))
+        return dict(diag=diag, pop=pop, iters=r.iterations, fp64_iters=r.fp64_iterations,
+                    trace=r.trace, idem=r.idem, gemm_seconds=r.gemm_seconds, gemm_flops=r.gemm_flops)
+
+    def kpe_stats(self):
+        n, s, b = C.c_int(), C.c_double(), C.c_double()
+        self._check(lib().ltb_kpe_stats(self._h, C.byref(n), C.byref(s), C.byref(b)))
+        return dict(launches=n.value, seconds=s.value, bytes=b.value)
+
+    def set_timing(self, on=True):
+        self._check(lib().ltb_set_kpe_timing(self._h, 1 if on else 0))
+
+    def set_tiling(self, chunk_cols=0, atoms_per_wave=0):
+        self._check(lib().ltb_set_kpe_tiling(self._h, chunk_cols, atoms_per_wave))

@@ -1,0 +1,98 @@
+// getKPE (withSP2_deltaH.cpp:1068-1185) on a block of vectors:
+//     Y = w_0 X + sum_{m=1}^{M-1} w_m T_m(H~) X,   w = (c_0/2, c_m g_m)
+// The block is norb x ld, row-major (columns contiguous), as in cheb.hip.  One
+// launch advances the recursion one step and accumulates theta += w_m t_m in
+// the same pass (the reference's MatMult + 6 BLAS-1 calls per step).
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "ctx.hpp"
+
+namespace {
+
+// MODE 0: t1 = H t0, theta = w0 t0 + w1 t1 ; MODE 2: t_{n+1} = 2 H t_n - t_{n-1}, theta += w t
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_apply_step(const int* __restrict__ rowptr,
+                                                    const int* __restrict__ col,
+                                                    const T* __restrict__ hv,
+                                                    const T* __restrict__ xcur,
+                                                    T* __restrict__ xnext,
+                                                    double* __restrict__ theta, int nat, int ld,
+                                                    double w0, double w1) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int A = blockIdx.y * 4 + wave;
+  const int c = blockIdx.x * 64 + lane;
+  if (A >= nat) return;
+  T acc[4] = {0, 0, 0, 0};
+  for (int k = rowptr[A]; k < rowptr[A + 1]; k++) {
+    const int B = col[k];
+    const T* h = hv + 16 * (size_t)k;
+    T x[4];
+    for (int s = 0; s < 4; s++) x[s] = xcur[(size_t)(4 * B + s) * ld + c];
+    for (int r = 0; r < 4; r++)
+      for (int s = 0; s < 4; s++) acc[r] = fma(h[4 * r + s], x[s], acc[r]);
+  }
+  for (int r = 0; r < 4; r++) {
+    const size_t o = (size_t)(4 * A + r) * ld + c;
+    T y;
+    if (MODE == 0) {
+      y = acc[r];
+      theta[o] = w0 * (double)xcur[o] + w1 * (double)y;
+    } else {
+      y = T(2) * acc[r] - xnext[o];
+      theta[o] += w1 * (double)y;
+    }
+    xnext[o] = y;
+  }
+}
+
+template <typename T>
+int run_apply(ltb_ctx* c, int M, const double* w, int ncols, const double* X, double* Y) {
+  const int ld = (ncols + 63) / 64 * 64;
+  const size_t n = (size_t)c->norb * ld;
+  std::vector<T> hx(n, T(0));
+  for (int j = 0; j < ncols; j++)
+    for (int r = 0; r < c->norb; r++) hx[(size_t)r * ld + j] = (T)X[(size_t)j * c->norb + r];
+  T *a = nullptr, *b = nullptr;
+  double* th = nullptr;
+  LTB_HIP(c, hipMalloc(&a, n * sizeof(T)));
+  LTB_HIP(c, hipMalloc(&b, n * sizeof(T)));
+  LTB_HIP(c, hipMalloc(&th, n * sizeof(double)));
+  LTB_HIP(c, hipMemcpyAsync(a, hx.data(), n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  const T* hv = sizeof(T) == 4 ? (const T*)c->d_ht32 : (const T*)c->d_ht64;
+  dim3 grid(ld / 64, (c->nat + 3) / 4), block(256);
+  // step 1: t1 = H t0 into b; theta = w0 t0 + w1 t1
+  hipLaunchKernelGGL((k_apply_step<T, 0>), grid, block, 0, c->stream, c->d_rowptr, c->d_col, hv, a,
+                     b, th, c->nat, ld, w[0], M > 1 ? w[1] : 0.0);
+  // steps m >= 2: cur alternates b, a; next overwrites the older one
+  for (int m = 2; m < M; m++) {
+    T* cur = (m & 1) ? a : b;
+    T* nxt = (m & 1) ? b : a;
+    hipLaunchKernelGGL((k_apply_step<T, 2>), grid, block, 0, c->stream, c->d_rowptr, c->d_col, hv,
+                       cur, nxt, th, c->nat, ld, 0.0, w[m]);
+  }
+  LTB_HIP(c, hipGetLastError());
+  std::vector<double> ht(n);
+  LTB_HIP(c, hipMemcpyAsync(ht.data(), th, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  LTB_HIP(c, hipStreamSynchronize(c->stream));
+  hipFree(a);
+  hipFree(b);
+  hipFree(th);
+  for (int j = 0; j < ncols; j++)
+    for (int r = 0; r < c->norb; r++) Y[(size_t)j * c->norb + r] = ht[(size_t)r * ld + j];
+  return LTB_OK;
+}
+
+}  // namespace
+
+extern "C" int ltb_kpe_apply(ltb_ctx* c, int M, const double* w, int ncols, const double* X,
+                             double* Y, int precision) {
+  if (!c) return LTB_ERR_ARG;
+  if (c->nat <= 0) return set_err(c, LTB_ERR_STATE, "no system loaded");
+  if (M < 1 || !w || ncols < 1 || !X || !Y) return set_err(c, LTB_ERR_ARG, "bad kpe_apply arguments");
+  if (precision == LTB_FP32_ACC64) return run_apply<float>(c, M, w, ncols, X, Y);
+  if (precision == LTB_FP64) return run_apply<double>(c, M, w, ncols, X, Y);
+  return set_err(c, LTB_ERR_ARG, "bad precision");
+}

@@ -1,0 +1,81 @@
+// Device context shared by the HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/ltb.h"
+
+// scalars that stay on the device so one SCC iteration needs no host round trip
+struct DevScalars {
+  double emin, emax;   // Gershgorin bounds of H_total
+  double a, b;         // rescale (H - b)/a
+  double mu;           // scaled chemical potential
+  double nelec;        // 2 * sum w_m mu_m
+  double eband;        // 2 Tr(P H_total)
+  double dq;           // |q_out - q_in|_inf (e)
+  int nonfinite;       // set by reductions that met NaN/Inf
+  int pad;
+};
+
+struct ltb_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int cu_count = 0;
+  // system
+  int nat = 0, norb = 0, nnzb = 0, gnnz = 0;
+  double nelectrons = 0;
+  int *d_rowptr = nullptr, *d_col = nullptr, *d_blkrow = nullptr;
+  double* d_h0 = nullptr;       // [nnzb][16]
+  double* d_ht64 = nullptr;     // rescaled
+  float* d_ht32 = nullptr;
+  double* d_offabs = nullptr;   // per orbital sum |H0_ij|, j != i
+  double* d_diag0 = nullptr;    // per orbital H0_ii
+  int *d_grow = nullptr, *d_gcol = nullptr;
+  double* d_gval = nullptr;
+  double *d_p0 = nullptr, *d_pin = nullptr, *d_pout = nullptr, *d_v = nullptr;
+  DevScalars* d_sc = nullptr;
+  double* d_red = nullptr;      // reduction scratch [4096]
+  // KPE buffers
+  void *d_xa = nullptr, *d_xb = nullptr;
+  size_t xabytes = 0, xbbytes = 0;
+  double* d_part = nullptr;
+  size_t partbytes = 0;
+  double* d_mu1 = nullptr;      // per column [T_1]_jj
+  size_t mu1bytes = 0;
+  double* d_mom = nullptr;      // [M+1][nat]
+  double* d_momtot = nullptr;   // [M+1]
+  int mom_M = -1;
+  size_t mombytes = 0;
+  // Fermi / density
+  double *d_W = nullptr, *d_X = nullptr, *d_g = nullptr, *d_c = nullptr, *d_w = nullptr;
+  int fd_K = 0, fd_M = 0;
+  // tiling
+  int chunk_cols = 0;       // 0 = auto
+  int atoms_per_wave = 0;   // 0 = auto
+  // statistics of the last ltb_kpe_moments
+  int stat_launches = 0;
+  double stat_seconds = 0, stat_bytes = 0;
+  bool timing = false;
+  std::vector<hipEvent_t> events;
+  // SP2 dense buffers
+  void *d_sa = nullptr, *d_sb = nullptr;
+  size_t sbytes = 0;
+};
+
+int set_err(ltb_ctx* c, int code, const std::string& msg);
+
+#define LTB_HIP(ctx, call)                                                              \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return set_err((ctx), LTB_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// helpers implemented in ltb_capi.hip, used by sp2.hip
+int ensure_buffer(ltb_ctx* c, void** p, size_t* cap, size_t bytes);
+int kpe_moments_impl(ltb_ctx* c, int M, int precision);
+int sp2_dense_impl(ltb_ctx* c, double Ne, int max_iter, int fp64_tail, double* diagP, double* pop,
+                   ltb_sp2_result* res);

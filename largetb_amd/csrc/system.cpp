@@ -1,0 +1,395 @@
+// Host-side assembly of one alpha-quartz tight-binding system:
+//   geometry        (initializeCoordinates, withSP2_deltaH.cpp:504-667)
+//   neighbour list  (getHaloAtoms :372-501, ModifyNList :355-370) -- here an
+//                   O(N) cell list with minimum images for large periodic
+//                   cells, and the reference's image-by-image scan otherwise
+//   H0 blocks       (get4by4HmatrixElements :211-349, constructH :969-1066)
+//   Gamma           (initializeForSCF :854-900)
+#include "system.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <set>
+
+namespace ltb {
+namespace {
+
+// fractional coordinates of the 9 atoms of the cell (:530-654); the second
+// Si sits at z = 0.6777 in the reference (:546-548) and is kept there
+const double kFrac[9][3] = {
+    {0.4697, 0.0000, 0.0000}, {0.0000, 0.4697, 0.6777}, {0.5303, 0.5303, 0.3333},
+    {0.4133, 0.2672, 0.1188}, {0.2672, 0.4133, 0.5479}, {0.7328, 0.1461, 0.7855},
+    {0.5867, 0.8539, 0.2145}, {0.8539, 0.5867, 0.4521}, {0.1461, 0.7328, 0.8812}};
+const int kZCell[9] = {14, 14, 14, 8, 8, 8, 8, 8, 8};
+
+// two-centre fit tables (:75-142); columns are (pp-sigma-like, pp-pi-like,
+// sp, ss) by their use at :224-253
+const double kSiSi[20][4] = {
+    {17.865, -5.7398, 17.262, -13.794},   {87.114, 5.8664, 39.685, -7.1328},
+    {-722.33, -52.111, -283.25, 27.816},  {66.059, -3.2651, 34.498, -27.631},
+    {-613.94, -6.6631, -41.298, 12.075},  {-727.27, 1.3757, -112.62, 5.9498},
+    {246.1, -84.273, 241.71, -164.58},    {-1564.3, 4.1541, -179.83, 22.449},
+    {-589.53, -7.9678, 215.3, -36.682},   {-1020.8, 33.73, 7.8809, -87.911},
+    {-812.85, -33.408, 13.727, 158.22},   {1189.7, 49.751, 54.842, -56.155},
+    {-96.931, 8.9531, 102.98, -91.347},   {860.35, 1.2204, -44.669, 12.487},
+    {3551.6, 9.5425, -32.771, 10.438},    {468.97, 1.6056, -128.73, -1.5888},
+    {-317.44, 6.9669, 133.21, 82.532},    {517.74, 47.052, -56.922, 1.1376},
+    {210.8, -18.241, -18.441, 80.163},    {-443.57, 36.135, 52.835, -13.413}};
+const double kOO[20][4] = {
+    {5.5526, -0.82118, 3.813, -1.2323},   {-31.886, 10.518, -41.071, 52.454},
+    {233.96, -39.064, 272.9, -407.19},    {-60.015, 61.253, -128.95, -294.22},
+    {213.73, -204.34, 410.72, 1263.6},    {49.054, 32.239, -78.906, -1060},
+    {-73.081, 132.93, -81.269, 1956.2},   {33.537, -13.278, -137.34, -384.47},
+    {78.858, 165.14, -273.61, -873.53},   {-167.41, 68.088, -277.49, 642.65},
+    {75.731, 7.9468, -767.04, -615.24},   {-63.795, -44.453, -288.57, 988.73},
+    {-78.765, 32.997, -76.092, -811.61},  {124.63, -144.7, -189.3, 554.51},
+    {40.615, -149.44, 780.52, 379.23},    {82.116, -29.881, 410.4, -93.407},
+    {-80.656, -54.736, 203.1, -131.08},   {-74.163, 16.814, 710.57, -213.88},
+    {7.3441, 20.02, -100.83, 128.52},     {-93.219, 6.9584, 267.84, -242.74}};
+const double kSiO[20][4] = {
+    {13.778, -2.7061, 8.6477, -7.6931},   {9.0798, 19.494, -22.43, 52.885},
+    {-15.173, -145.38, 262.06, -468.23},  {9.4007, -23.026, 223.86, -100.16},
+    {40.783, 126.17, -785.64, 228.63},    {23.249, -34.501, 334.88, -253.56},
+    {-17.829, -75.476, 156.18, -638.68},  {18.37, -22.003, 10.705, -312.46},
+    {1.7289, 75.972, 572.55, -1010.6},    {-30.424, 17.518, -55.884, -478.46},
+    {-31.663, -36.928, -349.49, -688.24}, {-82.425, -26.609, -390.25, -296.52},
+    {35.07, 77.948, -143.41, 157.81},     {-27.066, -22.899, -390.79, 580.81},
+    {-20.555, -140.91, -152.48, 401.59},  {49.585, -5.9967, 502.82, 768.76},
+    {-8.7806, 109.97, -113.52, 980.42},   {37.13, -18.019, -148.42, 243.34},
+    {-20.674, 15.652, 904.76, 317.15},    {16.513, 39.501, -630.1, 84.78}};
+
+// SplitMix64 stream -> Box-Muller pairs: a portable, documented generator so
+// the displaced structures are the same on every platform and compiler.
+struct Gauss {
+  uint64_t s;
+  bool have = false;
+  double spare = 0;
+  uint64_t next() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return (double)((next() >> 11) + 1) * 0x1.0p-53; }  // (0, 1]
+  double normal() {
+    if (have) {
+      have = false;
+      return spare;
+    }
+    double u1 = uniform(), u2 = uniform();
+    double r = std::sqrt(-2.0 * std::log(u1)), t = 2.0 * kPi * u2;
+    spare = r * std::sin(t);
+    have = true;
+    return r * std::cos(t);
+  }
+};
+
+struct Neighbor {
+  int j;
+  double d[3];  // r_j - r_i (image included)
+};
+
+Status fail(int code, std::string msg) { return Status{code, std::move(msg)}; }
+
+// inverse of a 3x3 matrix whose rows are the cell vectors
+bool invert3(const double a[9], double inv[9]) {
+  double c00 = a[4] * a[8] - a[5] * a[7], c01 = a[5] * a[6] - a[3] * a[8],
+         c02 = a[3] * a[7] - a[4] * a[6];
+  double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+  if (!(std::fabs(det) > 1e-12)) return false;
+  double id = 1.0 / det;
+  inv[0] = c00 * id;
+  inv[3] = c01 * id;
+  inv[6] = c02 * id;
+  inv[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+  inv[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+  inv[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+  inv[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+  inv[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+  inv[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+  return true;
+}
+
+// perpendicular width of the cell along row d
+double width(const double c[9], int d) {
+  int e = (d + 1) % 3, f = (d + 2) % 3;
+  double cr[3] = {c[3 * e + 1] * c[3 * f + 2] - c[3 * e + 2] * c[3 * f + 1],
+                  c[3 * e + 2] * c[3 * f + 0] - c[3 * e + 0] * c[3 * f + 2],
+                  c[3 * e + 0] * c[3 * f + 1] - c[3 * e + 1] * c[3 * f + 0]};
+  double vol = std::fabs(c[3 * d] * cr[0] + c[3 * d + 1] * cr[1] + c[3 * d + 2] * cr[2]);
+  return vol / std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+}
+
+constexpr double kRcut2 = 16.0;  // r^2 < 16 Bohr^2 (:392)
+constexpr double kRmin2 = 0.1;   // r^2 > 0.1 (:421)
+
+// Reference scan: every image (lp, mp, np) in loop order, every atom j.
+// Repeated j keeps the last image (INSERT_VALUES, constructH :1022).
+Status scan_images(const ltb_system& s, std::vector<std::vector<Neighbor>>& nb) {
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; d++) lo[d] = hi[d] = 0;
+  for (int d = 0; d < 3; d++)
+    if (s.periodic_mask >> d & 1) lo[d] = -1, hi[d] = 1;
+  const int n = s.natoms;
+  for (int i = 0; i < n; i++) {
+    std::vector<Neighbor>& out = nb[i];
+    int entries = 0;
+    for (int l = lo[0]; l <= hi[0]; l++)
+      for (int m = lo[1]; m <= hi[1]; m++)
+        for (int q = lo[2]; q <= hi[2]; q++) {
+          double sh[3];
+          for (int d = 0; d < 3; d++) sh[d] = l * s.cell[d] + m * s.cell[3 + d] + q * s.cell[6 + d];
+          for (int j = 0; j < n; j++) {
+            double dx = s.xyz[3 * j] + sh[0] - s.xyz[3 * i];
+            double dy = s.xyz[3 * j + 1] + sh[1] - s.xyz[3 * i + 1];
+            double dz = s.xyz[3 * j + 2] + sh[2] - s.xyz[3 * i + 2];
+            double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 <= kRmin2 && !(j == i && l == 0 && m == 0 && q == 0))
+              return fail(2, "atoms " + std::to_string(i) + " and " + std::to_string(j) +
+                                 " closer than sqrt(0.1) Bohr");
+            if (r2 < kRcut2 && r2 > kRmin2) {
+              if (++entries > kMaxNeighbors)
+                return fail(3, "atom " + std::to_string(i) + " has more than 20 neighbours");
+              auto it = std::find_if(out.begin(), out.end(), [&](const Neighbor& x) { return x.j == j; });
+              Neighbor nbv{j, {dx, dy, dz}};
+              if (it != out.end())
+                *it = nbv;
+              else
+                out.push_back(nbv);
+            }
+          }
+        }
+  }
+  return Status::ok();
+}
+
+// Cell list with minimum images; valid when every periodic direction is
+// wider than twice the cutoff (then at most one image of j is in range).
+Status cell_list(const ltb_system& s, const double inv[9], std::vector<std::vector<Neighbor>>& nb) {
+  const int n = s.natoms;
+  std::vector<double> f(3 * (size_t)n);
+  double fmin[3] = {1e300, 1e300, 1e300}, fmax[3] = {-1e300, -1e300, -1e300};
+  for (int i = 0; i < n; i++)
+    for (int d = 0; d < 3; d++) {
+      double v = s.xyz[3 * i] * inv[d] + s.xyz[3 * i + 1] * inv[3 + d] + s.xyz[3 * i + 2] * inv[6 + d];
+      f[3 * i + d] = v;
+      fmin[d] = std::min(fmin[d], v);
+      fmax[d] = std::max(fmax[d], v);
+    }
+  int nbin[3];
+  double w[3], f0[3], span[3];
+  for (int d = 0; d < 3; d++) {
+    w[d] = width(s.cell, d);
+    bool per = s.periodic_mask >> d & 1;
+    f0[d] = per ? 0.0 : fmin[d];
+    span[d] = per ? 1.0 : std::max(fmax[d] - fmin[d], 1e-9);
+    nbin[d] = std::max(1, (int)std::floor(w[d] * span[d] / 4.0));
+    nbin[d] = std::min(nbin[d], 1024);
+  }
+  auto bin_of = [&](int i, int d) {
+    bool per = s.periodic_mask >> d & 1;
+    double v = (f[3 * i + d] - f0[d]) / span[d];
+    if (per) v -= std::floor(v);
+    int b = (int)std::floor(v * nbin[d]);
+    return std::min(std::max(b, 0), nbin[d] - 1);
+  };
+  size_t nb_tot = (size_t)nbin[0] * nbin[1] * nbin[2];
+  std::vector<int> head(nb_tot + 1, 0), items(n);
+  std::vector<int> bid(n);
+  for (int i = 0; i < n; i++) {
+    bid[i] = (bin_of(i, 0) * nbin[1] + bin_of(i, 1)) * nbin[2] + bin_of(i, 2);
+    head[bid[i] + 1]++;
+  }
+  for (size_t b = 0; b < nb_tot; b++) head[b + 1] += head[b];
+  {
+    std::vector<int> fill(head.begin(), head.end() - 1);
+    for (int i = 0; i < n; i++) items[fill[bid[i]]++] = i;
+  }
+  for (int i = 0; i < n; i++) {
+    int bi[3] = {bin_of(i, 0), bin_of(i, 1), bin_of(i, 2)};
+    std::set<size_t> bins;
+    for (int o0 = -1; o0 <= 1; o0++)
+      for (int o1 = -1; o1 <= 1; o1++)
+        for (int o2 = -1; o2 <= 1; o2++) {
+          int b[3] = {bi[0] + o0, bi[1] + o1, bi[2] + o2};
+          bool ok = true;
+          for (int d = 0; d < 3; d++) {
+            if (s.periodic_mask >> d & 1)
+              b[d] = ((b[d] % nbin[d]) + nbin[d]) % nbin[d];
+            else if (b[d] < 0 || b[d] >= nbin[d])
+              ok = false;
+          }
+          if (ok) bins.insert(((size_t)b[0] * nbin[1] + b[1]) * nbin[2] + b[2]);
+        }
+    std::vector<Neighbor>& out = nb[i];
+    for (size_t b : bins)
+      for (int k = head[b]; k < head[b + 1]; k++) {
+        int j = items[k];
+        if (j == i) continue;
+        double df[3];
+        for (int d = 0; d < 3; d++) {
+          df[d] = f[3 * j + d] - f[3 * i + d];
+          if (s.periodic_mask >> d & 1) df[d] -= std::nearbyint(df[d]);
+        }
+        double dv[3];
+        for (int d = 0; d < 3; d++) dv[d] = df[0] * s.cell[d] + df[1] * s.cell[3 + d] + df[2] * s.cell[6 + d];
+        double r2 = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
+        if (r2 <= kRmin2)
+          return fail(2, "atoms " + std::to_string(i) + " and " + std::to_string(j) +
+                             " closer than sqrt(0.1) Bohr");
+        if (r2 < kRcut2) {
+          if ((int)out.size() >= kMaxNeighbors)
+            return fail(3, "atom " + std::to_string(i) + " has more than 20 neighbours");
+          out.push_back(Neighbor{j, {dv[0], dv[1], dv[2]}});
+        }
+      }
+  }
+  return Status::ok();
+}
+
+}  // namespace
+
+// get4by4HmatrixElements (:211-349).  The radial sums
+// S_c(r) = sum_{p=1..20} tab[p][c] * cos(p acos(r/10)) * exp(-p r)
+// are formed once per pair; the angular factors follow :224-253.
+Status sk_block(int zi, int zj, double x, double y, double z, double h[16]) {
+  double r2 = x * x + y * y + z * z, r = std::sqrt(r2);
+  const double(*tab)[4] = nullptr;
+  for (int k = 0; k < 16; k++) h[k] = 0;
+  if ((zi == 14 && zj == 8) || (zi == 8 && zj == 14)) {
+    if (r < 0.01) return fail(2, "Si and O closer than 0.01 Bohr");
+    tab = kSiO;
+  } else if (zi == 14 && zj == 14) {
+    if (r < 0.01) {  // onsite (:258-263)
+      h[0] = -0.39572506;
+      h[5] = h[10] = h[15] = -0.15031380;
+      return Status::ok();
+    }
+    tab = kSiSi;
+  } else if (zi == 8 && zj == 8) {
+    if (r < 0.01) {  // onsite (:299-304)
+      h[0] = -0.87883246;
+      h[5] = h[10] = h[15] = -0.33213167;
+      return Status::ok();
+    }
+    tab = kOO;
+  } else {
+    return fail(2, "unknown atom pair " + std::to_string(zi) + "-" + std::to_string(zj));
+  }
+  double S[4] = {0, 0, 0, 0};
+  const double u = std::acos(r / 10);
+  for (int p = 1; p <= 20; p++) {
+    double t = std::cos(p * u) * std::exp(-p * r);
+    for (int c = 0; c < 4; c++) S[c] += tab[p - 1][c] * t;
+  }
+  const double d[3] = {x, y, z};
+  h[0] = S[3];
+  for (int a = 0; a < 3; a++) {
+    h[1 + a] = S[2] * d[a] / r;
+    h[4 * (1 + a)] = -S[2] * d[a] / r;
+    for (int b = 0; b < 3; b++) {
+      double dd = d[a] * d[b] / r2;
+      h[4 * (1 + a) + 1 + b] = S[0] * dd - S[1] * (dd - (a == b ? 1.0 : 0.0));
+    }
+  }
+  return Status::ok();
+}
+
+Status build_quartz(int L, int M, int N, int lattice, double sigma_ang, uint64_t seed,
+                    ltb_system& s) {
+  if (L <= 0 || M <= 0 || N <= 0 || (long long)L * M * N * 9 > (1LL << 28))
+    return fail(1, "bad supercell size");
+  if (lattice != 0 && lattice != 1) return fail(1, "bad lattice mode");
+  double a, c;
+  if (lattice == 1) {
+    a = 9.725;  // Bohr (:508)
+    c = 10.7;
+  } else {
+    a = 4.913 * kAngToBohr;
+    c = 5.405 * kAngToBohr;
+  }
+  const double R[3][3] = {{a, 0.0, 0.0}, {-a * std::sin(kPi / 6), a * std::cos(kPi / 6), 0.0}, {0.0, 0.0, c}};
+  s.natoms = 9 * L * M * N;
+  s.Z.assign(s.natoms, 0);
+  s.xyz.assign(3 * (size_t)s.natoms, 0.0);
+  int at = 0;
+  for (int i = 0; i < L; i++)
+    for (int j = 0; j < M; j++)
+      for (int k = 0; k < N; k++) {
+        double o[3];
+        for (int d = 0; d < 3; d++) o[d] = i * R[0][d] + j * R[1][d] + k * R[2][d];
+        for (int t = 0; t < 9; t++, at++) {
+          for (int d = 0; d < 3; d++)
+            s.xyz[3 * at + d] = o[d] + (kFrac[t][0] * R[0][d] + kFrac[t][1] * R[1][d] + kFrac[t][2] * R[2][d]);
+          s.Z[at] = kZCell[t];
+        }
+      }
+  if (lattice == 0 && sigma_ang > 0) {
+    Gauss g{seed};
+    for (size_t q = 0; q < s.xyz.size(); q++) s.xyz[q] += sigma_ang * kAngToBohr * g.normal();
+  }
+  for (int d = 0; d < 3; d++) {
+    s.cell[d] = L * R[0][d];
+    s.cell[3 + d] = M * R[1][d];
+    s.cell[6 + d] = N * R[2][d];
+  }
+  // reference lattice: no periodic images along z (:400-403)
+  s.periodic_mask = lattice == 1 ? 3 : 7;
+  return build_from_coords(s);
+}
+
+Status build_from_coords(ltb_system& s) {
+  const int n = s.natoms;
+  if (n <= 0 || (int)s.Z.size() != n || s.xyz.size() != 3 * (size_t)n) return fail(1, "bad coordinates");
+  for (int i = 0; i < n; i++)
+    if (s.Z[i] != 14 && s.Z[i] != 8) return fail(2, "atomic number must be 14 (Si) or 8 (O)");
+  for (size_t q = 0; q < s.xyz.size(); q++)
+    if (!std::isfinite(s.xyz[q])) return fail(4, "non-finite coordinate");
+  double inv[9];
+  bool have_inv = invert3(s.cell, inv);
+  if (s.periodic_mask && !have_inv) return fail(1, "singular periodic cell");
+  bool large = have_inv;
+  for (int d = 0; d < 3; d++)
+    if ((s.periodic_mask >> d & 1) && width(s.cell, d) <= 2.0 * 4.0 + 1e-9) large = false;
+  std::vector<std::vector<Neighbor>> nb(n);
+  Status st = large ? cell_list(s, inv, nb) : scan_images(s, nb);
+  if (st.code) return st;
+
+  s.h_rowptr.assign(n + 1, 0);
+  s.g_rowptr.assign(n + 1, 0);
+  s.h_col.clear();
+  s.h_val.clear();
+  s.g_col.clear();
+  s.g_val.clear();
+  double blk[16];
+  for (int i = 0; i < n; i++) {
+    std::vector<Neighbor>& v = nb[i];
+    std::sort(v.begin(), v.end(), [](const Neighbor& x, const Neighbor& y) { return x.j < y.j; });
+    st = sk_block(s.Z[i], s.Z[i], 0, 0, 0, blk);
+    if (st.code) return st;
+    s.h_col.push_back(i);
+    s.h_val.insert(s.h_val.end(), blk, blk + 16);
+    s.g_col.push_back(i);
+    s.g_val.push_back(-1.0);  // -Us, Us = 1.0 (:876, :887)
+    for (const Neighbor& q : v) {
+      st = sk_block(s.Z[i], s.Z[q.j], q.d[0], q.d[1], q.d[2], blk);
+      if (st.code) return st;
+      s.h_col.push_back(q.j);
+      s.h_val.insert(s.h_val.end(), blk, blk + 16);
+      double r = std::sqrt(q.d[0] * q.d[0] + q.d[1] * q.d[1] + q.d[2] * q.d[2]);
+      s.g_col.push_back(q.j);
+      s.g_val.push_back(1.0 / r);  // (:890-896)
+    }
+    s.h_rowptr[i + 1] = (int)s.h_col.size();
+    s.g_rowptr[i + 1] = (int)s.g_col.size();
+  }
+  s.p0.resize(n);
+  s.nelectrons = 0;
+  for (int i = 0; i < n; i++) {
+    s.p0[i] = s.Z[i] == 14 ? 2.0 : 3.0;  // per-spin valence: Si 4/2, O 6/2
+    s.nelectrons += 2.0 * s.p0[i];
+  }
+  return Status::ok();
+}
+}  // namespace ltb

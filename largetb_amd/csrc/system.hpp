@@ -1,0 +1,40 @@
+// Host-side structure of one tight-binding system: geometry, neighbour list,
+// H0 as BSR 4x4 (FP64) and the Gamma matrix (CSR, FP64).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+struct ltb_system {
+  int natoms = 0;
+  std::vector<int> Z;         // 14 Si / 8 O
+  std::vector<double> xyz;    // Bohr, [natoms][3]
+  double cell[9] = {0};       // periodic vectors (rows), Bohr
+  int periodic_mask = 0;
+  // H0: BSR, diagonal block first in each row, then ascending columns
+  std::vector<int> h_rowptr, h_col;
+  std::vector<double> h_val;  // [nnzb][16], row-major 4x4
+  // Gamma: CSR, same row layout
+  std::vector<int> g_rowptr, g_col;
+  std::vector<double> g_val;
+  double nelectrons = 0;
+  std::vector<double> p0;     // neutral per-spin populations
+};
+
+namespace ltb {
+constexpr int kMaxNeighbors = 20;  // maxN_A, withSP2_deltaH.cpp:26
+constexpr double kAngToBohr = 1.88973;
+constexpr double kPi = 3.141592653589793238463;
+
+// status + message; status 0 = OK
+struct Status {
+  int code = 0;
+  std::string msg;
+  static Status ok() { return {}; }
+};
+
+Status build_quartz(int L, int M, int N, int lattice, double sigma_ang, uint64_t seed,
+                    ltb_system& s);
+Status build_from_coords(ltb_system& s);  // uses s.Z, s.xyz, s.cell, s.periodic_mask
+Status sk_block(int zi, int zj, double x, double y, double z, double h[16]);
+}  // namespace ltb

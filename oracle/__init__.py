@@ -1,0 +1,299 @@
+"""ctypes front end of the CPU FP64 oracle (oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+MAXN = 20
+MODE_CONFIG = 0
+MODE_REFERENCE = 1
+HARTREE_EV = 27.211386
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        _setup(_lib)
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+PI = C.POINTER(C.c_int)
+PD = C.POINTER(C.c_double)
+
+
+class KpeParams(C.Structure):
+    _fields_ = [("M", C.c_int), ("kT", C.c_double), ("K", C.c_int), ("Ne", C.c_double),
+                ("bisect_iters", C.c_int)]
+
+
+class KpeResult(C.Structure):
+    _fields_ = [("mu", C.c_double), ("mu_ha", C.c_double), ("a", C.c_double), ("b", C.c_double),
+                ("emin", C.c_double), ("emax", C.c_double), ("nelec", C.c_double),
+                ("eband", C.c_double)]
+
+
+def _setup(L):
+    L.orc_quartz.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, PD, PI, PD]
+    L.orc_neighbors.argtypes = [C.c_int, PD, PD, C.c_int, PI, PI, PD]
+    L.orc_sk_block.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, PD]
+    L.orc_build_h.argtypes = [C.c_int, PI, PI, PI, PD, C.c_int, PI, PI, PD]
+    L.orc_build_gamma.argtypes = [C.c_int, PI, PI, PD, C.c_double, C.c_int, PI, PI, PD]
+    L.orc_gershgorin.argtypes = [C.c_int, PI, PI, PD, PD, PD, PD]
+    L.orc_gmM.argtypes = [C.c_int, PD]
+    L.orc_gmM.restype = None
+    L.orc_jackson.argtypes = [C.c_int, PD]
+    L.orc_jackson.restype = None
+    L.orc_cheb_nodes.argtypes = [C.c_int, PD]
+    L.orc_cheb_nodes.restype = None
+    L.orc_step_coeffs.argtypes = [C.c_int, C.c_double, PD]
+    L.orc_step_coeffs.restype = None
+    L.orc_fd_coeffs.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, PD]
+    L.orc_fd_coeffs.restype = None
+    L.orc_rescaled_csr.argtypes = [C.c_int, PI, PI, PD, PD, C.c_double, C.c_double, PI, PI, PD]
+    L.orc_kpe_diag_moments.argtypes = [C.c_int, PI, PI, PD, C.c_int, C.c_int, C.c_int, PD, C.c_int]
+    L.orc_kpe_apply.argtypes = [C.c_int, PI, PI, PD, C.c_int, PD, PD, PD]
+    L.orc_count.argtypes = [C.c_int, PD, PD]
+    L.orc_count.restype = C.c_double
+    L.orc_density.argtypes = [C.c_int, PI, PI, PD, PD, C.POINTER(KpeParams), PD, PD,
+                              C.POINTER(KpeResult), C.c_int]
+    L.orc_scc.argtypes = [C.c_int, PI, PI, PD, PI, PI, PD, PD, C.POINTER(KpeParams), C.c_double,
+                          C.c_double, C.c_int, PD, PI, PD, C.POINTER(KpeResult), C.c_int]
+    L.orc_sp2_dense.argtypes = [C.c_int, PD, C.c_double, C.c_int, C.c_double, PD]
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _check(st, what):
+    if st != 0:
+        raise OracleError(f"{what} failed with status {st}")
+
+
+def quartz(L, M, N, mode=MODE_CONFIG, sigma_ang=0.0, seed=1):
+    nat = 9 * L * M * N
+    xyz = np.zeros(3 * nat)
+    Z = np.zeros(nat, np.int32)
+    cell = np.zeros(9)
+    _check(lib().orc_quartz(L, M, N, mode, sigma_ang, seed, _p(xyz, C.c_double), _p(Z, C.c_int),
+                            _p(cell, C.c_double)), "orc_quartz")
+    return xyz.reshape(nat, 3), Z, cell.reshape(3, 3)
+
+
+class System:
+    """Geometry + neighbour list + H (BSR 4x4) + Gamma (CSR), all FP64."""
+
+    def __init__(self, xyz, Z, cell, periodic_mask=7, Us=1.0):
+        self.xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        self.Z = np.ascontiguousarray(Z, np.int32)
+        self.cell = np.ascontiguousarray(cell, np.float64).reshape(3, 3)
+        nat = self.nat = len(self.Z)
+        self.norb = 4 * nat
+        nn = np.zeros(nat, np.int32)
+        nbr = np.zeros(nat * MAXN, np.int32)
+        nd = np.zeros(nat * MAXN * 3)
+        _check(lib().orc_neighbors(nat, _p(self.xyz, C.c_double), _p(self.cell, C.c_double),
+                                   periodic_mask, _p(nn, C.c_int), _p(nbr, C.c_int),
+                                   _p(nd, C.c_double)), "orc_neighbors")
+        self.nn, self.nbr, self.nd = nn, nbr, nd
+        cap = nat * (MAXN + 1)
+        rp = np.zeros(nat + 1, np.int32)
+        col = np.zeros(cap, np.int32)
+        val = np.zeros(cap * 16)
+        nz = lib().orc_build_h(nat, _p(self.Z, C.c_int), _p(nn, C.c_int), _p(nbr, C.c_int),
+                               _p(nd, C.c_double), cap, _p(rp, C.c_int), _p(col, C.c_int),
+                               _p(val, C.c_double))
+        if nz < 0:
+            raise OracleError(f"orc_build_h failed with status {-nz}")
+        self.h_rowptr, self.h_col, self.h_val = rp, col[:nz].copy(), val[:nz * 16].copy()
+        grp = np.zeros(nat + 1, np.int32)
+        gcol = np.zeros(cap, np.int32)
+        gval = np.zeros(cap)
+        nz = lib().orc_build_gamma(nat, _p(nn, C.c_int), _p(nbr, C.c_int), _p(nd, C.c_double), Us,
+                                   cap, _p(grp, C.c_int), _p(gcol, C.c_int), _p(gval, C.c_double))
+        if nz < 0:
+            raise OracleError(f"orc_build_gamma failed with status {-nz}")
+        self.g_rowptr, self.g_col, self.g_val = grp, gcol[:nz].copy(), gval[:nz].copy()
+        self.nelectrons = float(np.sum(np.where(self.Z == 14, 4.0, 6.0)))
+        self.p0 = np.where(self.Z == 14, 2.0, 3.0)  # neutral per-spin populations
+
+    @classmethod
+    def quartz(cls, L, M, N, mode=MODE_CONFIG, sigma_ang=0.0, seed=1):
+        xyz, Z, cell = quartz(L, M, N, mode, sigma_ang, seed)
+        return cls(xyz, Z, cell, periodic_mask=(3 if mode == MODE_REFERENCE else 7))
+
+    @property
+    def nnzb(self):
+        return len(self.h_col)
+
+    def dense_h(self, shift=None):
+        n = self.norb
+        H = np.zeros((n, n))
+        v = self.h_val.reshape(-1, 4, 4)
+        for i in range(self.nat):
+            for k in range(self.h_rowptr[i], self.h_rowptr[i + 1]):
+                j = self.h_col[k]
+                H[4 * i:4 * i + 4, 4 * j:4 * j + 4] = v[k]
+        if shift is not None:
+            H[np.diag_indices(n)] += shift
+        return H
+
+    def dense_gamma(self):
+        G = np.zeros((self.nat, self.nat))
+        for i in range(self.nat):
+            for k in range(self.g_rowptr[i], self.g_rowptr[i + 1]):
+                G[i, self.g_col[k]] = self.g_val[k]
+        return G
+
+    def gershgorin(self, shift=None):
+        lo, hi = C.c_double(), C.c_double()
+        sp = _p(np.ascontiguousarray(shift, np.float64), C.c_double) if shift is not None else None
+        _check(lib().orc_gershgorin(self.nat, _p(self.h_rowptr, C.c_int), _p(self.h_col, C.c_int),
+                                    _p(self.h_val, C.c_double), sp, C.byref(lo), C.byref(hi)),
+               "orc_gershgorin")
+        return lo.value, hi.value
+
+    def rescaled_csr(self, a, b, shift=None):
+        n = self.norb
+        nnz = self.nnzb * 16
+        crow = np.zeros(n + 1, np.int32)
+        ccol = np.zeros(nnz, np.int32)
+        cval = np.zeros(nnz)
+        sp = None
+        if shift is not None:
+            shift = np.ascontiguousarray(shift, np.float64)
+            sp = _p(shift, C.c_double)
+        lib().orc_rescaled_csr(self.nat, _p(self.h_rowptr, C.c_int), _p(self.h_col, C.c_int),
+                               _p(self.h_val, C.c_double), sp, a, b, _p(crow, C.c_int),
+                               _p(ccol, C.c_int), _p(cval, C.c_double))
+        return crow, ccol, cval
+
+    def diag_moments(self, a, b, nm, j0=0, j1=None, shift=None, nthreads=0):
+        j1 = self.norb if j1 is None else j1
+        crow, ccol, cval = self.rescaled_csr(a, b, shift)
+        out = np.zeros((j1 - j0) * nm)
+        _check(lib().orc_kpe_diag_moments(self.norb, _p(crow, C.c_int), _p(ccol, C.c_int),
+                                          _p(cval, C.c_double), nm, j0, j1, _p(out, C.c_double),
+                                          nthreads), "orc_kpe_diag_moments")
+        return out.reshape(j1 - j0, nm)
+
+    def kpe_apply(self, a, b, w, x, shift=None):
+        crow, ccol, cval = self.rescaled_csr(a, b, shift)
+        w = np.ascontiguousarray(w, np.float64)
+        x = np.ascontiguousarray(x, np.float64)
+        out = np.zeros(self.norb)
+        _check(lib().orc_kpe_apply(self.norb, _p(crow, C.c_int), _p(ccol, C.c_int),
+                                   _p(cval, C.c_double), len(w), _p(w, C.c_double),
+                                   _p(x, C.c_double), _p(out, C.c_double)), "orc_kpe_apply")
+        return out
+
+    def kpe_params(self, M, kT_ev=0.05, K=8192, bisect_iters=100):
+        kT = kT_ev / HARTREE_EV if kT_ev > 0 else 0.0
+        return KpeParams(M, kT, K, self.nelectrons, bisect_iters)
+
+    def density(self, M, shift=None, kT_ev=0.05, K=8192, nthreads=0, want_moments=False):
+        p = self.kpe_params(M, kT_ev, K)
+        pop = np.zeros(self.nat)
+        res = KpeResult()
+        mom = np.zeros(self.nat * (M + 1)) if want_moments else None
+        sp = None
+        if shift is not None:
+            shift = np.ascontiguousarray(shift, np.float64)
+            sp = _p(shift, C.c_double)
+        _check(lib().orc_density(self.nat, _p(self.h_rowptr, C.c_int), _p(self.h_col, C.c_int),
+                                 _p(self.h_val, C.c_double), sp, C.byref(p), _p(pop, C.c_double),
+                                 _p(mom, C.c_double) if mom is not None else None, C.byref(res),
+                                 nthreads), "orc_density")
+        out = dict(pop=pop, mu=res.mu, mu_ha=res.mu_ha, a=res.a, b=res.b, emin=res.emin,
+                   emax=res.emax, nelec=res.nelec, eband=res.eband)
+        if mom is not None:
+            out["moments"] = mom.reshape(self.nat, M + 1)
+        return out
+
+    def scc(self, M, guess=None, kT_ev=0.05, K=8192, mix=0.2, tol=1e-6, maxit=200, nthreads=0):
+        p = self.kpe_params(M, kT_ev, K)
+        pio = np.ascontiguousarray(self.p0 if guess is None else guess, np.float64).copy()
+        its = C.c_int()
+        hist = np.zeros(maxit)
+        res = KpeResult()
+        st = lib().orc_scc(self.nat, _p(self.h_rowptr, C.c_int), _p(self.h_col, C.c_int),
+                           _p(self.h_val, C.c_double), _p(self.g_rowptr, C.c_int),
+                           _p(self.g_col, C.c_int), _p(self.g_val, C.c_double),
+                           _p(self.p0, C.c_double), C.byref(p), mix, tol, maxit,
+                           _p(pio, C.c_double), C.byref(its), _p(hist, C.c_double),
+                           C.byref(res), nthreads)
+        return dict(status=st, pop=pio, q=2 * pio, iters=its.value, hist=hist[:its.value],
+                    mu=res.mu, mu_ha=res.mu_ha, a=res.a, b=res.b, eband=res.eband,
+                    nelec=res.nelec)
+
+
+def gmM(M):
+    g = np.zeros(M)
+    lib().orc_gmM(M, _p(g, C.c_double))
+    return g
+
+
+def jackson(N):
+    g = np.zeros(N)
+    lib().orc_jackson(N, _p(g, C.c_double))
+    return g
+
+
+def cheb_nodes(n):
+    x = np.zeros(n)
+    lib().orc_cheb_nodes(n, _p(x, C.c_double))
+    return x
+
+
+def step_coeffs(M, mu):
+    c = np.zeros(M)
+    lib().orc_step_coeffs(M, mu, _p(c, C.c_double))
+    return c
+
+
+def fd_coeffs(M, mu, kT, K=8192):
+    c = np.zeros(M)
+    lib().orc_fd_coeffs(M, mu, kT, K, _p(c, C.c_double))
+    return c
+
+
+def kpe_weights(c, g):
+    w = c * g
+    w[0] = c[0] / 2.0
+    return w
+
+
+def sk_block(zi, zj, d):
+    h = np.zeros(16)
+    _check(lib().orc_sk_block(zi, zj, float(d[0]), float(d[1]), float(d[2]), _p(h, C.c_double)),
+           "orc_sk_block")
+    return h.reshape(4, 4)
+
+
+def sp2_dense(X, nocc, maxit=100, idem_tol=1e-12):
+    X = np.ascontiguousarray(X, np.float64).copy()
+    idem = C.c_double()
+    it = lib().orc_sp2_dense(X.shape[0], _p(X, C.c_double), nocc, maxit, idem_tol, C.byref(idem))
+    if it < 0:
+        raise OracleError("orc_sp2_dense failed")
+    return X, it, idem.value

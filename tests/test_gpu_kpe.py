@@ -1,0 +1,178 @@
+"""HIP path against the oracle, through the C ABI (libltb.so).
+
+Tolerances: FP64 paths agree with the FP64 oracle to rounding (stated per
+test); the FP32 recursion (FP32 vectors and H~, FP64 moment accumulation) is
+held to 5e-6 per atom moment and 2e-6 e per charge."""
+import numpy as np
+import pytest
+
+import largetb_amd as L
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def pair(cells, sigma=0.05, seed=1):
+    return L.System.quartz(*cells, sigma_ang=sigma, seed=seed), O.System.quartz(*cells, sigma_ang=sigma, seed=seed)
+
+
+@pytest.fixture(scope="module")
+def c0():
+    return pair((2, 2, 2))
+
+
+def oracle_atom_moments(o, a, b, nm, shift=None):
+    m = o.diag_moments(a, b, nm, shift=shift)  # [orb][nm]
+    return m.reshape(o.nat, 4, nm).sum(1).T     # [nm][nat]
+
+
+def prepare(ctx, s, p=None):
+    ctx.load(s)
+    ctx.set_populations(p)
+    lo, hi = ctx.gershgorin()
+    a, b = (hi - lo) / 2, (hi + lo) / 2
+    ctx.rescale(a, b)
+    return a, b
+
+
+def test_gershgorin_and_shift(ctx, c0):
+    s, o = c0
+    ctx.load(s)
+    ctx.set_populations(None)
+    assert np.abs(ctx.shift()).max() == 0.0
+    assert np.allclose(ctx.gershgorin(), o.gershgorin(), rtol=0, atol=1e-14)
+    rng = np.random.default_rng(1)
+    p = o.p0 + 0.1 * rng.standard_normal(o.nat)
+    ctx.set_populations(p)
+    v = o.dense_gamma() @ (p - o.p0)
+    assert np.abs(ctx.shift() - v).max() < 1e-14
+    assert np.allclose(ctx.gershgorin(), o.gershgorin(np.repeat(v, 4)), rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("M", [200, 51])
+def test_moments_fp64_match_oracle(ctx, c0, M):
+    s, o = c0
+    a, b = prepare(ctx, s)
+    am, tot = ctx.kpe_moments(M, L.FP64)
+    ref = oracle_atom_moments(o, a, b, M + 1)
+    assert np.abs(am - ref).max() < 1e-11
+    assert np.abs(tot - ref.sum(1)).max() < 1e-9
+    assert np.all(am[0] == 4.0)
+
+
+def test_moments_fp32_within_stated_bound(ctx, c0):
+    s, o = c0
+    a, b = prepare(ctx, s)
+    am, _ = ctx.kpe_moments(200, L.FP32_ACC64)
+    ref = oracle_atom_moments(o, a, b, 201)
+    assert np.abs(am - ref).max() < 5e-6
+
+
+@pytest.mark.parametrize("prec", [L.FP64, L.FP32_ACC64])
+def test_ragged_chunks_equal_single_chunk(ctx, prec):
+    s = L.System.quartz(3, 3, 3, sigma_ang=0.1, seed=2)  # 972 orbitals
+    prepare(ctx, s)
+    try:
+        ctx.set_tiling(0, 0)
+        ref, _ = ctx.kpe_moments(64, prec)
+        ctx.set_tiling(128, 1)  # 8 chunks, the last one 76 columns wide
+        got, _ = ctx.kpe_moments(64, prec)
+        ctx.set_tiling(320, 3)
+        got2, _ = ctx.kpe_moments(64, prec)
+    finally:
+        ctx.set_tiling(0, 0)
+    tol = 1e-12 if prec == L.FP64 else 2e-6
+    assert np.abs(got - ref).max() < tol
+    assert np.abs(got2 - ref).max() < tol
+
+
+def test_density_matches_oracle(ctx, c0):
+    s, o = c0
+    prepare(ctx, s)
+    ctx.kpe_moments(200, L.FP64)
+    d = ctx.density(200)
+    r = o.density(200)
+    assert d["nelec"] == pytest.approx(384.0, abs=1e-8)
+    assert np.abs(d["pop"] - r["pop"]).max() < 1e-9
+    assert d["eband"] == pytest.approx(r["eband"], abs=1e-8)
+    assert abs(d["mu"] - r["mu"]) < 1e-6  # mu in a 6.6 eV gap is weakly determined
+
+
+def test_density_step_function_mode(ctx, c0):
+    s, o = c0
+    prepare(ctx, s)
+    ctx.kpe_moments(200, L.FP64)
+    d = ctx.density(200, kT_ev=0.0)
+    r = o.density(200, kT_ev=0.0)
+    assert np.abs(d["pop"] - r["pop"]).max() < 1e-9
+
+
+def test_kpe_apply_matches_getKPE(ctx, c0):
+    s, o = c0
+    a, b = prepare(ctx, s)
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((3, s.norb))
+    c = O.fd_coeffs(200, -0.2, 1e-3, 8192)
+    w = O.kpe_weights(c, O.gmM(200))
+    Y = ctx.kpe_apply(200, w, X, L.FP64)
+    for j in range(3):
+        ref = o.kpe_apply(a, b, w, X[j])
+        assert np.abs(Y[j] - ref).max() < 1e-11
+    Y32 = ctx.kpe_apply(200, w, X, L.FP32_ACC64)
+    assert np.abs(Y32 - Y).max() < 1e-4
+
+
+def test_scc_fp64_matches_oracle(ctx, c0):
+    s, o = c0
+    ctx.load(s)
+    g = ctx.scc(200, precision=L.FP64, tol=1e-6, max_iter=300)
+    r = o.scc(200, tol=1e-6, maxit=300)
+    assert g["status"] == 0 and r["status"] == 0
+    assert abs(g["iters"] - r["iters"]) <= 1
+    assert np.abs(g["q"] - r["q"]).max() < 1e-8
+    assert g["eband"] == pytest.approx(r["eband"], abs=1e-7)
+    assert 2 * g["pop"].sum() == pytest.approx(384.0, abs=1e-8)
+
+
+def test_scc_mixed_precision_matches_fp64(ctx, c0):
+    s, o = c0
+    ctx.load(s)
+    ref = ctx.scc(200, precision=L.FP64, tol=1e-6, max_iter=300)
+    mix = ctx.scc(200, precision=L.FP32_ACC64, switch_tol=1e-4, tol=1e-6, max_iter=300)
+    assert mix["status"] == 0
+    assert mix["fp64_iters"] >= 1
+    assert np.abs(mix["q"] - ref["q"]).max() < 1e-6
+
+
+def test_scc_iteration_cap_reports_not_converged(ctx, c0):
+    s, _ = c0
+    ctx.load(s)
+    r = ctx.scc(200, precision=L.FP64, max_iter=3, check=False)
+    assert r["status"] == L.LTB_NOT_CONVERGED and r["iters"] == 3
+
+
+def test_nonfinite_populations_are_reported(ctx, c0):
+    s, o = c0
+    ctx.load(s)
+    p = o.p0.copy()
+    p[5] = np.nan
+    ctx.set_populations(p)
+    with pytest.raises(L.LtbError) as e:
+        ctx.gershgorin()
+    assert e.value.status == L.LTB_ERR_NONFINITE
+    ctx.set_populations(o.p0)
+
+
+def test_c1_size_properties(ctx):
+    """4,608 atoms (18,432 orbitals): FP32 vs FP64 and sum rules at full size."""
+    s = L.System.quartz(8, 8, 8, sigma_ang=0.05, seed=1)
+    prepare(ctx, s)
+    M = 64
+    am64, tot64 = ctx.kpe_moments(M, L.FP64)
+    am32, tot32 = ctx.kpe_moments(M, L.FP32_ACC64)
+    assert tot64[0] == s.norb
+    assert np.all(np.abs(am64) <= 4 + 1e-9)          # |[T_m]_jj| <= 1
+    assert np.abs(am32 - am64).max() < 5e-6
+    d = ctx.density(M)
+    assert 2 * d["pop"].sum() == pytest.approx(s.nelectrons, rel=1e-12)
+    assert d["nelec"] == pytest.approx(s.nelectrons, rel=1e-12)
