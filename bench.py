@@ -73,6 +73,7 @@ def main():
     ap.add_argument("--M", type=int, default=512)
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU sharding is design-only "
@@ -86,15 +87,24 @@ def main():
     log(f"structure {args.cells}: {s.nat} atoms, {s.nnzb} blocks, {time.perf_counter() - t0:.2f} s")
     ctx = L.Context(0)
     ctx.load(s)
+    tiling = None
+    if not args.no_autotune:
+        ctx.set_populations(None)
+        lo, hi = ctx.gershgorin()
+        ctx.rescale((hi - lo) / 2, (hi + lo) / 2)
+        t0 = time.perf_counter()
+        tiling, gbs = ctx.autotune(prec, trial_steps=3)
+        log(f"autotune {time.perf_counter() - t0:.2f} s: {tiling} at {gbs:.0f} GB/s")
+    else:
+        tiling = ctx.get_tiling(prec)
     if args.warmup > 0:
         r = ctx.scc(args.M, precision=prec, max_iter=args.warmup, tol=1e-30, check=False)
         log(f"warmup {args.warmup} iteration(s): {r['seconds']:.3f} s")
     ctx.set_timing(True)
-    import torch
-    torch.cuda.synchronize()
+    # every C-ABI call returns after its stream has drained, so the wall clock
+    # around ltb_scc_run brackets exactly the timed device work
     t0 = time.perf_counter()
     r = ctx.scc(args.M, precision=prec, max_iter=args.steps, tol=1e-30, check=False)
-    torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     st = ctx.kpe_stats()
     log(f"timed {args.steps} iteration(s): {wall:.3f} s, dq={r['dq']:.3e}")
@@ -114,7 +124,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_cheb_step", "launches": st["launches"],
-                     "mean_launch_us": round(st["seconds"] / max(st["launches"], 1) * 1e6, 2)},
+                     "mean_launch_us": round(st["seconds"] / max(st["launches"], 1) * 1e6, 2),
+                     "tiling": tiling},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cells, args.sigma, args.seed, args.M)

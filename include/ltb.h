@@ -185,9 +185,29 @@ int ltb_kpe_stats(ltb_ctx* ctx, int* launches, double* step_seconds, double* alg
  * stream) so ltb_kpe_stats reports their summed device time. */
 int ltb_set_kpe_timing(ltb_ctx* ctx, int on);
 
-/* Tuning knobs of the Chebyshev kernel (0 = keep current / autotuned):
- * columns in flight per chunk and atoms per wave. */
-int ltb_set_kpe_tiling(ltb_ctx* ctx, int chunk_cols, int atoms_per_wave);
+/* Work mapping of the Chebyshev block-SpMM kernel.  No reference counterpart
+ * (the reference runs one vector at a time through PETSc's MatMult). */
+enum ltb_kpe_schedule {
+  LTB_SCHED_AUTO = 0,
+  LTB_SCHED_BLOCKED = 1, /* workgroup = (column group, consecutive atoms)          */
+  LTB_SCHED_SWEEP = 2    /* resident grid; each XCD sweeps its column groups       */
+};
+typedef struct {
+  int chunk_cols;     /* unit-vector columns in flight per chunk (0 = auto)      */
+  int cols_per_lane;  /* V in {1, 2, 4}: a wave covers 64*V columns (0 = auto)   */
+  int schedule;       /* enum ltb_kpe_schedule                                   */
+  int blocks_per_cu;  /* sweep: resident workgroups per CU (0 = auto)            */
+  int atoms_per_wave; /* blocked: consecutive atoms per wave (0 = auto)          */
+  int nontemporal;    /* 1: t_{n-1}, t_{n+1} bypass the caches; 0: cached; -1 auto */
+} ltb_kpe_tiling;
+int ltb_set_kpe_tiling(ltb_ctx* ctx, const ltb_kpe_tiling* t);
+/* the tiling the kernel would use for the loaded system (auto fields resolved) */
+int ltb_get_kpe_tiling(ltb_ctx* ctx, int precision, ltb_kpe_tiling* out);
+/* Time `trial_steps` recursion steps of every candidate tiling on the loaded
+ * system, keep the fastest (set on the context) and report it with its
+ * algorithmic GB/s. */
+int ltb_kpe_autotune(ltb_ctx* ctx, int precision, int trial_steps, ltb_kpe_tiling* best,
+                     double* best_gbs);
 
 #ifdef __cplusplus
 }

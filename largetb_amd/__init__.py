@@ -50,6 +50,17 @@ class DensityResult(C.Structure):
                 ("nelec", C.c_double), ("eband", C.c_double)]
 
 
+class KpeTiling(C.Structure):
+    _fields_ = [("chunk_cols", C.c_int), ("cols_per_lane", C.c_int), ("schedule", C.c_int),
+                ("blocks_per_cu", C.c_int), ("atoms_per_wave", C.c_int), ("nontemporal", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+SCHED_AUTO, SCHED_BLOCKED, SCHED_SWEEP = 0, 1, 2
+
+
 class SccParams(C.Structure):
     _fields_ = [("M", C.c_int), ("kT", C.c_double), ("K", C.c_int), ("mix", C.c_double),
                 ("tol", C.c_double), ("max_iter", C.c_int), ("precision", C.c_int),
@@ -96,7 +107,9 @@ SIGNATURES = {
     "ltb_sp2_dense": (C.c_int, [VP, C.c_double, C.c_int, C.c_int, PD, PD, C.POINTER(Sp2Result)]),
     "ltb_kpe_stats": (C.c_int, [VP, PI, PD, PD]),
     "ltb_set_kpe_timing": (C.c_int, [VP, C.c_int]),
-    "ltb_set_kpe_tiling": (C.c_int, [VP, C.c_int, C.c_int]),
+    "ltb_set_kpe_tiling": (C.c_int, [VP, C.POINTER(KpeTiling)]),
+    "ltb_get_kpe_tiling": (C.c_int, [VP, C.c_int, C.POINTER(KpeTiling)]),
+    "ltb_kpe_autotune": (C.c_int, [VP, C.c_int, C.c_int, C.POINTER(KpeTiling), PD]),
 }
 
 
@@ -307,5 +320,18 @@ class Context:
     def set_timing(self, on=True):
         self._check(lib().ltb_set_kpe_timing(self._h, 1 if on else 0))
 
-    def set_tiling(self, chunk_cols=0, atoms_per_wave=0):
-        self._check(lib().ltb_set_kpe_tiling(self._h, chunk_cols, atoms_per_wave))
+    def set_tiling(self, chunk_cols=0, cols_per_lane=0, schedule=SCHED_AUTO, blocks_per_cu=0,
+                   atoms_per_wave=0, nontemporal=-1):
+        t = KpeTiling(chunk_cols, cols_per_lane, schedule, blocks_per_cu, atoms_per_wave, nontemporal)
+        self._check(lib().ltb_set_kpe_tiling(self._h, C.byref(t)))
+
+    def get_tiling(self, precision=FP32_ACC64):
+        t = KpeTiling()
+        self._check(lib().ltb_get_kpe_tiling(self._h, precision, C.byref(t)))
+        return t.as_dict()
+
+    def autotune(self, precision=FP32_ACC64, trial_steps=3):
+        t = KpeTiling()
+        g = C.c_double()
+        self._check(lib().ltb_kpe_autotune(self._h, precision, trial_steps, C.byref(t), C.byref(g)))
+        return t.as_dict(), g.value

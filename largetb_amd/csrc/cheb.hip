@@ -12,13 +12,25 @@
 // they give [T_{2n+1}]_jj and [T_{2n+2}]_jj, so M moments take M/2 launches.
 //
 // Work mapping (CDNA4, wave64): a wave owns one atom (block row) at a time
-// and 64*V columns; every lane holds V consecutive columns, so each row read
-// or write of a wave is one contiguous 64*V*sizeof(T) segment.  The atom's
-// block-row pointers, column indices and 4x4 H~ blocks are wave-uniform and
-// come through scalar loads.  A workgroup is 4 waves (4*apw atoms, same
-// columns); its per-column dot partials are summed in LDS and written once
-// per workgroup, then folded by k_cheb_reduce.  Workgroup ids are remapped so
-// that each XCD sweeps a contiguous range of (column group, atom range).
+// and W = 64*V columns (a "column group"); every lane holds V consecutive
+// columns, so each row access of a wave is one contiguous W*sizeof(T)
+// segment.  Block-row pointers, column indices and the 4x4 blocks of H~ are
+// wave-uniform (scalar loads).  The rows of up to 4 neighbour blocks and the
+// t_{n-1} rows are all requested before the first FMA.  t_{n-1} and t_{n+1}
+// are touched once per step and may bypass the caches (non-temporal); the
+// t_n rows are re-read by every neighbouring atom and should hit L2.
+//
+// Two schedules:
+//  * blocked: workgroup (column group, 4*apw consecutive atoms), ids remapped
+//    so each XCD gets a contiguous range;
+//  * sweep: a resident grid of 8*slots workgroups; workgroup b runs on XCD
+//    b%8 and that XCD owns column groups xcd, xcd+8, ...; its waves sweep the
+//    atoms of one column group cyclically (wave w takes atoms w, w+S, ...), so
+//    the rows in flight on an XCD form one compact window and the t_n rows a
+//    neighbour re-reads (a few hundred to ~1300 atoms away) are still in that
+//    XCD's L2.
+// Per-column dot partials are summed in LDS per workgroup, written once, and
+// folded into per-atom moments by k_cheb_reduce.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,54 +41,151 @@
 namespace {
 
 template <typename T, int V>
-struct VecT;
-template <>
-struct VecT<float, 4> {
-  using type = float4;
+struct VecT {
+  typedef T type __attribute__((ext_vector_type(V)));
 };
-template <>
-struct VecT<float, 2> {
-  using type = float2;
-};
-template <>
-struct VecT<float, 1> {
-  using type = float;
-};
-template <>
-struct VecT<double, 2> {
-  using type = double2;
-};
-template <>
-struct VecT<double, 1> {
-  using type = double;
+template <typename T>
+struct VecT<T, 1> {
+  typedef T type;
 };
 
-template <typename T, int V>
+template <typename T, int V, bool NT>
 __device__ __forceinline__ void load_row(const T* p, T (&x)[V]) {
   using W = typename VecT<T, V>::type;
-  W w = *reinterpret_cast<const W*>(p);
+  W w;
+  if (NT)
+    w = __builtin_nontemporal_load(reinterpret_cast<const W*>(p));
+  else
+    w = *reinterpret_cast<const W*>(p);
   __builtin_memcpy(x, &w, sizeof(W));
 }
 
-template <typename T, int V>
+template <typename T, int V, bool NT>
 __device__ __forceinline__ void store_row(T* p, const T (&x)[V]) {
   using W = typename VecT<T, V>::type;
   W w;
   __builtin_memcpy(&w, x, sizeof(W));
-  *reinterpret_cast<W*>(p) = w;
+  if (NT)
+    __builtin_nontemporal_store(w, reinterpret_cast<W*>(p));
+  else
+    *reinterpret_cast<W*>(p) = w;
 }
 
+// One block row (atom A) of one recursion step for the lane's V columns.
 // MODE 0: t_1 = H~ t_0 (t_0 implicit identity columns, nothing read)
 // MODE 1: t_2 = 2 H~ t_1 - t_0 (t_0 implicit)
 // MODE 2: t_{n+1} = 2 H~ t_n - t_{n-1}, t_{n-1} read from and overwritten in xnext
-template <typename T, int V, int MODE>
-__global__ __launch_bounds__(256) void k_cheb_step(const int* __restrict__ rowptr,
-                                                   const int* __restrict__ col,
-                                                   const T* __restrict__ hv,
-                                                   const T* __restrict__ xcur,
-                                                   T* __restrict__ xnext, int nat, int ld, int j0,
-                                                   int nvalid, int apw, int n_sg,
-                                                   double* __restrict__ part) {
+template <typename T, int V, int MODE, bool NT>
+__device__ __forceinline__ void atom_step(const int* __restrict__ rowptr,
+                                          const int* __restrict__ col, const T* __restrict__ hv,
+                                          const T* __restrict__ xcur, T* __restrict__ xnext, int A,
+                                          int ld, int j0, int c0, int nvalid, T (&sxy)[V],
+                                          T (&syy)[V]) {
+  constexpr int G = 4;
+  const int kb = rowptr[A], ke = rowptr[A + 1];
+  T* out = xnext + (size_t)(4 * A) * ld + c0;
+  T prev[4][V];
+  if (MODE == 2) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) load_row<T, V, NT>(out + (size_t)r * ld, prev[r]);
+  }
+  T acc[4][V], xa[4][V];
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int v = 0; v < V; v++) acc[r][v] = T(0);
+  for (int k0 = kb; k0 < ke; k0 += G) {
+    T x[G][4][V];
+#pragma unroll
+    for (int u = 0; u < G; u++) {
+      if (k0 + u < ke) {
+        const int B = col[k0 + u];
+        if (MODE == 0) {
+#pragma unroll
+          for (int s = 0; s < 4; s++)
+#pragma unroll
+            for (int v = 0; v < V; v++)
+              x[u][s][v] = (4 * B + s == j0 + c0 + v && c0 + v < nvalid) ? T(1) : T(0);
+        } else {
+#pragma unroll
+          for (int s = 0; s < 4; s++)
+            load_row<T, V, false>(xcur + (size_t)(4 * B + s) * ld + c0, x[u][s]);
+        }
+      }
+    }
+    if (k0 == kb) {
+#pragma unroll
+      for (int s = 0; s < 4; s++)
+#pragma unroll
+        for (int v = 0; v < V; v++) xa[s][v] = x[0][s][v];
+    }
+#pragma unroll
+    for (int u = 0; u < G; u++) {
+      if (k0 + u < ke) {
+        const T* h = hv + 16 * (size_t)(k0 + u);
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+          for (int s = 0; s < 4; s++) {
+            const T hrs = h[4 * r + s];
+#pragma unroll
+            for (int v = 0; v < V; v++) acc[r][v] = fma(hrs, x[u][s][v], acc[r][v]);
+          }
+      }
+    }
+  }
+  T y[4][V];
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      if (MODE == 0)
+        y[r][v] = acc[r][v];
+      else if (MODE == 1)
+        y[r][v] = T(2) * acc[r][v] - ((4 * A + r == j0 + c0 + v && c0 + v < nvalid) ? T(1) : T(0));
+      else
+        y[r][v] = T(2) * acc[r][v] - prev[r][v];
+    }
+#pragma unroll
+  for (int r = 0; r < 4; r++) store_row<T, V, NT>(out + (size_t)r * ld, y[r]);
+#pragma unroll
+  for (int v = 0; v < V; v++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      sxy[v] = fma(y[r][v], xa[r][v], sxy[v]);
+      syy[v] = fma(y[r][v], y[r][v], syy[v]);
+    }
+}
+
+// sum the 4 waves' per-column partials in LDS (FP64) and write one row pair
+template <typename T, int V>
+__device__ __forceinline__ void flush_partials(const T (&sxy)[V], const T (&syy)[V], int wave,
+                                               int lane, double* __restrict__ part, size_t row2,
+                                               int ld, int colbase) {
+  constexpr int W = 64 * V;
+  __shared__ double sh[4][2][W];
+#pragma unroll
+  for (int v = 0; v < V; v++) {
+    sh[wave][0][lane * V + v] = (double)sxy[v];
+    sh[wave][1][lane * V + v] = (double)syy[v];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 2 * W; e += 256) {
+    const int which = e / W, cc = e - which * W;
+    const double s = (sh[0][which][cc] + sh[1][which][cc]) + (sh[2][which][cc] + sh[3][which][cc]);
+    part[(row2 + which) * ld + colbase + cc] = s;
+  }
+  __syncthreads();
+}
+
+template <typename T, int V, int MODE, bool NT>
+__global__ __launch_bounds__(256) void k_cheb_blocked(const int* __restrict__ rowptr,
+                                                      const int* __restrict__ col,
+                                                      const T* __restrict__ hv,
+                                                      const T* __restrict__ xcur,
+                                                      T* __restrict__ xnext, int nat, int ld,
+                                                      int j0, int nvalid, int apw, int n_sg,
+                                                      double* __restrict__ part) {
   constexpr int W = 64 * V;
   int bid = blockIdx.x;
   const int nb = gridDim.x;
@@ -88,109 +197,57 @@ __global__ __launch_bounds__(256) void k_cheb_step(const int* __restrict__ rowpt
   const int c0 = cg * W + lane * V;
   const int a_begin = (sg * 4 + wave) * apw;
   const int a_end = min(a_begin + apw, nat);
-
   T sxy[V], syy[V];
 #pragma unroll
   for (int v = 0; v < V; v++) sxy[v] = syy[v] = T(0);
+  for (int A = a_begin; A < a_end; ++A)
+    atom_step<T, V, MODE, NT>(rowptr, col, hv, xcur, xnext, A, ld, j0, c0, nvalid, sxy, syy);
+  flush_partials<T, V>(sxy, syy, wave, lane, part, (size_t)sg * 2, ld, cg * W);
+}
 
-  for (int A = a_begin; A < a_end; ++A) {
-    const int kb = rowptr[A], ke = rowptr[A + 1];
-    T acc[4][V], xa[4][V];
+template <typename T, int V, int MODE, bool NT>
+__global__ __launch_bounds__(256) void k_cheb_sweep(const int* __restrict__ rowptr,
+                                                    const int* __restrict__ col,
+                                                    const T* __restrict__ hv,
+                                                    const T* __restrict__ xcur,
+                                                    T* __restrict__ xnext, int nat, int ld, int j0,
+                                                    int nvalid, int n_cg, int slots,
+                                                    double* __restrict__ part) {
+  constexpr int W = 64 * V;
+  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int S = 4 * slots;
+  const int wid = 4 * slot + wave;
+  for (int cg = xcd; cg < n_cg; cg += 8) {
+    const int c0 = cg * W + lane * V;
+    T sxy[V], syy[V];
 #pragma unroll
-    for (int r = 0; r < 4; r++)
-#pragma unroll
-      for (int v = 0; v < V; v++) acc[r][v] = T(0);
-    for (int k = kb; k < ke; ++k) {
-      const int B = col[k];
-      const T* h = hv + 16 * (size_t)k;
-      T x[4][V];
-      if (MODE == 0) {
-#pragma unroll
-        for (int s = 0; s < 4; s++)
-#pragma unroll
-          for (int v = 0; v < V; v++)
-            x[s][v] = (4 * B + s == j0 + c0 + v && c0 + v < nvalid) ? T(1) : T(0);
-      } else {
-#pragma unroll
-        for (int s = 0; s < 4; s++) load_row<T, V>(xcur + (size_t)(4 * B + s) * ld + c0, x[s]);
-      }
-      if (k == kb) {
-#pragma unroll
-        for (int s = 0; s < 4; s++)
-#pragma unroll
-          for (int v = 0; v < V; v++) xa[s][v] = x[s][v];
-      }
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-#pragma unroll
-        for (int s = 0; s < 4; s++) {
-          const T hrs = h[4 * r + s];
-#pragma unroll
-          for (int v = 0; v < V; v++) acc[r][v] = fma(hrs, x[s][v], acc[r][v]);
-        }
-    }
-    T* out = xnext + (size_t)(4 * A) * ld + c0;
-    T y[4][V];
-    if (MODE == 0) {
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-#pragma unroll
-        for (int v = 0; v < V; v++) y[r][v] = acc[r][v];
-    } else if (MODE == 1) {
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-#pragma unroll
-        for (int v = 0; v < V; v++)
-          y[r][v] = T(2) * acc[r][v] - ((4 * A + r == j0 + c0 + v && c0 + v < nvalid) ? T(1) : T(0));
-    } else {
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        T p[V];
-        load_row<T, V>(out + (size_t)r * ld, p);
-#pragma unroll
-        for (int v = 0; v < V; v++) y[r][v] = T(2) * acc[r][v] - p[v];
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < 4; r++) store_row<T, V>(out + (size_t)r * ld, y[r]);
-#pragma unroll
-    for (int v = 0; v < V; v++) {
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        sxy[v] = fma(y[r][v], xa[r][v], sxy[v]);
-        syy[v] = fma(y[r][v], y[r][v], syy[v]);
-      }
-    }
-  }
-
-  __shared__ double sh[4][2][W];
-#pragma unroll
-  for (int v = 0; v < V; v++) {
-    sh[wave][0][lane * V + v] = (double)sxy[v];
-    sh[wave][1][lane * V + v] = (double)syy[v];
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < 2 * W; e += 256) {
-    const int which = e / W, cc = e - which * W;
-    double s = sh[0][which][cc] + sh[1][which][cc] + sh[2][which][cc] + sh[3][which][cc];
-    part[((size_t)sg * 2 + which) * ld + cg * W + cc] = s;
+    for (int v = 0; v < V; v++) sxy[v] = syy[v] = T(0);
+    for (int A = wid; A < nat; A += S)
+      atom_step<T, V, MODE, NT>(rowptr, col, hv, xcur, xnext, A, ld, j0, c0, nvalid, sxy, syy);
+    flush_partials<T, V>(sxy, syy, wave, lane, part, (size_t)(xcd * slots + slot) * 2, ld, cg * W);
   }
 }
 
 // Fold the workgroup partials of step n into per-atom moments:
 //   n == 0: mu_1 = <t_1, t_0>,           mu_2 = 2 <t_1, t_1> - 1
 //   n >= 1: mu_{2n+1} = 2 <t_{n+1}, t_n> - mu_1,  mu_{2n+2} = 2 |t_{n+1}|^2 - 1
-// and sum the 4 columns (orbitals) of each atom.  mom is [M+1][nat].
-__global__ __launch_bounds__(256) void k_cheb_reduce(const double* __restrict__ part, int n_sg,
-                                                     int ld, int nvalid, int j0, int n, int M,
+// and sum the 4 columns (orbitals) of each atom.  mom is [M+1][nat].  The
+// partial rows holding column c are rows [base(c), base(c) + count): blocked
+// schedule base 0, count n_sg; sweep schedule base (c/W % 8)*slots, count slots.
+__global__ __launch_bounds__(256) void k_cheb_reduce(const double* __restrict__ part, int count,
+                                                     int sweep_W, int slots, int ld, int nvalid,
+                                                     int j0, int n, int M,
                                                      double* __restrict__ mu1col,
                                                      double* __restrict__ mom, int nat) {
   const int lane = threadIdx.x & 63, slice = threadIdx.x >> 6;
   const int c = blockIdx.x * 64 + lane;
+  const int base = sweep_W > 0 ? ((c / sweep_W) & 7) * slots : 0;
   double sxy = 0, syy = 0;
-  for (int s = slice; s < n_sg; s += 4) {
-    sxy += part[(size_t)(2 * s) * ld + c];
-    syy += part[(size_t)(2 * s + 1) * ld + c];
+  for (int s = slice; s < count; s += 4) {
+    sxy += part[(size_t)(2 * (base + s)) * ld + c];
+    syy += part[(size_t)(2 * (base + s) + 1) * ld + c];
   }
   __shared__ double sh[2][4][64];
   sh[0][slice][lane] = sxy;
@@ -237,77 +294,122 @@ __global__ __launch_bounds__(256) void k_mom_total(const double* __restrict__ mo
 }
 
 struct Plan {
-  int V, W, ld, apw, n_sg;
+  int V, W, ld, schedule, nt;
+  int apw, n_sg;   // blocked
+  int slots;       // sweep
+  size_t part_rows;
 };
 
 template <typename T>
 Plan make_plan(const ltb_ctx* c) {
-  Plan p;
+  const KpeTiling& t = c->tiling;
+  Plan p{};
   const bool big = c->norb >= 4096;
-  if (sizeof(T) == 4)
-    p.V = big ? 4 : 1;
-  else
-    p.V = big ? 2 : 1;
+  p.V = t.cols_per_lane > 0 ? t.cols_per_lane : (big ? (sizeof(T) == 4 ? 2 : 1) : 1);
+  if (p.V != 1 && p.V != 2 && p.V != 4) p.V = 1;
   p.W = 64 * p.V;
-  int want = c->chunk_cols > 0 ? c->chunk_cols : (sizeof(T) == 4 ? 4096 : 2048);
+  int want = t.chunk_cols > 0 ? t.chunk_cols : (sizeof(T) == 4 ? 4096 : 2048);
   int ld = std::min(want, c->norb);
   p.ld = (ld + p.W - 1) / p.W * p.W;
-  int n_cg = p.ld / p.W;
-  if (c->atoms_per_wave > 0) {
-    p.apw = c->atoms_per_wave;
+  const int n_cg = p.ld / p.W;
+  p.schedule = t.schedule;
+  if (p.schedule == LTB_SCHED_AUTO) p.schedule = (n_cg >= 16 && big) ? LTB_SCHED_SWEEP : LTB_SCHED_BLOCKED;
+  if (p.schedule == LTB_SCHED_SWEEP && n_cg < 8) p.schedule = LTB_SCHED_BLOCKED;
+  p.nt = t.nontemporal < 0 ? 1 : t.nontemporal;
+  const int cus = std::max(c->cu_count, 8);
+  if (p.schedule == LTB_SCHED_SWEEP) {
+    int bpc = t.blocks_per_cu > 0 ? t.blocks_per_cu : 4;
+    p.slots = std::max(1, bpc * cus / 8);
+    p.part_rows = (size_t)8 * p.slots;
   } else {
-    // enough workgroups to fill the chip ~8 deep, but few partials per column
-    int target_blocks = 8 * std::max(c->cu_count, 1);
-    int sg_want = std::max(1, (target_blocks + n_cg - 1) / n_cg);
-    p.apw = std::max(1, (c->nat + 4 * sg_want - 1) / (4 * sg_want));
+    if (t.atoms_per_wave > 0) {
+      p.apw = t.atoms_per_wave;
+    } else {
+      int target_blocks = 8 * cus;
+      int sg_want = std::max(1, (target_blocks + n_cg - 1) / n_cg);
+      p.apw = std::max(1, (c->nat + 4 * sg_want - 1) / (4 * sg_want));
+    }
+    p.n_sg = (c->nat + 4 * p.apw - 1) / (4 * p.apw);
+    p.part_rows = p.n_sg;
   }
-  p.n_sg = (c->nat + 4 * p.apw - 1) / (4 * p.apw);
   return p;
 }
 
+template <typename T, int V, int MODE, bool NT>
+void launch_one(const ltb_ctx* c, const T* hv, const T* xcur, T* xnext, int ldc, int j0, int nvalid,
+                const Plan& p) {
+  const int n_cg = ldc / (64 * V);
+  if (p.schedule == LTB_SCHED_SWEEP) {
+    hipLaunchKernelGGL((k_cheb_sweep<T, V, MODE, NT>), dim3(8 * p.slots), dim3(256), 0, c->stream,
+                       c->d_rowptr, c->d_col, hv, xcur, xnext, c->nat, ldc, j0, nvalid, n_cg,
+                       p.slots, c->d_part);
+  } else {
+    hipLaunchKernelGGL((k_cheb_blocked<T, V, MODE, NT>), dim3(n_cg * p.n_sg), dim3(256), 0,
+                       c->stream, c->d_rowptr, c->d_col, hv, xcur, xnext, c->nat, ldc, j0, nvalid,
+                       p.apw, p.n_sg, c->d_part);
+  }
+}
+
 template <typename T, int V, int MODE>
-void launch_step(const ltb_ctx* c, const T* hv, const T* xcur, T* xnext, int ld, int j0, int nvalid,
-                 const Plan& p) {
-  const int n_cg = ld / (64 * V);
-  dim3 grid(n_cg * p.n_sg), block(256);
-  hipLaunchKernelGGL((k_cheb_step<T, V, MODE>), grid, block, 0, c->stream, c->d_rowptr, c->d_col,
-                     hv, xcur, xnext, c->nat, ld, j0, nvalid, p.apw, p.n_sg, c->d_part);
+void launch_nt(const ltb_ctx* c, const T* hv, const T* xcur, T* xnext, int ldc, int j0, int nvalid,
+               const Plan& p) {
+  if (p.nt)
+    launch_one<T, V, MODE, true>(c, hv, xcur, xnext, ldc, j0, nvalid, p);
+  else
+    launch_one<T, V, MODE, false>(c, hv, xcur, xnext, ldc, j0, nvalid, p);
 }
 
 template <typename T, int V>
-void launch_mode(const ltb_ctx* c, int mode, const T* hv, const T* xcur, T* xnext, int ld, int j0,
+void launch_mode(const ltb_ctx* c, int mode, const T* hv, const T* xcur, T* xnext, int ldc, int j0,
                  int nvalid, const Plan& p) {
   if (mode == 0)
-    launch_step<T, V, 0>(c, hv, xcur, xnext, ld, j0, nvalid, p);
+    launch_nt<T, V, 0>(c, hv, xcur, xnext, ldc, j0, nvalid, p);
   else if (mode == 1)
-    launch_step<T, V, 1>(c, hv, xcur, xnext, ld, j0, nvalid, p);
+    launch_nt<T, V, 1>(c, hv, xcur, xnext, ldc, j0, nvalid, p);
   else
-    launch_step<T, V, 2>(c, hv, xcur, xnext, ld, j0, nvalid, p);
+    launch_nt<T, V, 2>(c, hv, xcur, xnext, ldc, j0, nvalid, p);
+}
+
+template <typename T>
+void launch_step(const ltb_ctx* c, int mode, const T* hv, const T* xcur, T* xnext, int ldc, int j0,
+                 int nvalid, const Plan& p) {
+  if (p.V == 4)
+    launch_mode<T, 4>(c, mode, hv, xcur, xnext, ldc, j0, nvalid, p);
+  else if (p.V == 2)
+    launch_mode<T, 2>(c, mode, hv, xcur, xnext, ldc, j0, nvalid, p);
+  else
+    launch_mode<T, 1>(c, mode, hv, xcur, xnext, ldc, j0, nvalid, p);
+}
+
+template <typename T>
+int ensure_buffers(ltb_ctx* c, const Plan& p, int M) {
+  size_t xb = (size_t)c->norb * p.ld * sizeof(T);
+  int st;
+  if ((st = ensure_buffer(c, &c->d_xa, &c->xabytes, xb))) return st;
+  if ((st = ensure_buffer(c, &c->d_xb, &c->xbbytes, xb))) return st;
+  if ((st = ensure_buffer(c, (void**)&c->d_part, &c->partbytes, p.part_rows * 2 * p.ld * sizeof(double))))
+    return st;
+  if ((st = ensure_buffer(c, (void**)&c->d_mu1, &c->mu1bytes, (size_t)p.ld * sizeof(double)))) return st;
+  if (M >= 0) {
+    if ((st = ensure_buffer(c, (void**)&c->d_mom, &c->mombytes,
+                            (size_t)(M + 1) * c->nat * sizeof(double) + (size_t)(M + 1) * sizeof(double))))
+      return st;
+    c->d_momtot = c->d_mom + (size_t)(M + 1) * c->nat;
+  }
+  return LTB_OK;
 }
 
 template <typename T>
 int run_moments(ltb_ctx* c, int M) {
   const Plan p = make_plan<T>(c);
   const int nsteps = (M + 1) / 2;  // step n yields moments up to 2n+2
-  size_t xb = (size_t)c->norb * p.ld * sizeof(T);
-  int st;
-  if ((st = ensure_buffer(c, &c->d_xa, &c->xabytes, xb))) return st;
-  if ((st = ensure_buffer(c, &c->d_xb, &c->xbbytes, xb))) return st;
-  if ((st = ensure_buffer(c, (void**)&c->d_part, &c->partbytes,
-                          (size_t)p.n_sg * 2 * p.ld * sizeof(double))))
-    return st;
-  if ((st = ensure_buffer(c, (void**)&c->d_mu1, &c->mu1bytes, (size_t)p.ld * sizeof(double))))
-    return st;
-  if ((st = ensure_buffer(c, (void**)&c->d_mom, &c->mombytes,
-                          (size_t)(M + 1) * c->nat * sizeof(double) + (size_t)(M + 1) * sizeof(double))))
-    return st;
-  c->d_momtot = c->d_mom + (size_t)(M + 1) * c->nat;
+  int st = ensure_buffers<T>(c, p, M);
+  if (st) return st;
   const T* hv = sizeof(T) == 4 ? (const T*)c->d_ht32 : (const T*)c->d_ht64;
   T* A = (T*)c->d_xa;
   T* B = (T*)c->d_xb;
-
   const int nlaunch_est = ((c->norb + p.ld - 1) / p.ld) * nsteps;
-  if (c->timing && (int)c->events.size() < 2 * nlaunch_est) {
+  if (c->timing) {
     while ((int)c->events.size() < 2 * nlaunch_est) {
       hipEvent_t e;
       LTB_HIP(c, hipEventCreate(&e));
@@ -320,24 +422,36 @@ int run_moments(ltb_ctx* c, int M) {
   for (int j0 = 0; j0 < c->norb; j0 += p.ld) {
     const int nvalid = std::min(p.ld, c->norb - j0);
     const int ldc = (nvalid + p.W - 1) / p.W * p.W;
+    const int n_cg = ldc / p.W;
+    // a narrow last chunk may leave XCDs idle under the sweep schedule
+    Plan pc = p;
+    if (pc.schedule == LTB_SCHED_SWEEP && n_cg < 8) {
+      pc.schedule = LTB_SCHED_BLOCKED;
+      int target_blocks = 8 * std::max(c->cu_count, 8);
+      int sg_want = std::max(1, (target_blocks + n_cg - 1) / n_cg);
+      pc.apw = std::max(1, (c->nat + 4 * sg_want - 1) / (4 * sg_want));
+      pc.n_sg = (c->nat + 4 * pc.apw - 1) / (4 * pc.apw);
+      if ((size_t)pc.n_sg > p.part_rows) {
+        pc.n_sg = (int)p.part_rows;
+        pc.apw = (c->nat + 4 * pc.n_sg - 1) / (4 * pc.n_sg);
+        pc.n_sg = (c->nat + 4 * pc.apw - 1) / (4 * pc.apw);
+      }
+    }
+    const bool sweep = pc.schedule == LTB_SCHED_SWEEP;
     for (int n = 0; n < nsteps; n++) {
       const int mode = n == 0 ? 0 : (n == 1 ? 1 : 2);
       T* cur = (n & 1) ? B : A;
       T* nxt = (n == 0) ? B : ((n & 1) ? A : B);
       if (c->timing) LTB_HIP(c, hipEventRecord(c->events[2 * launches], c->stream));
-      if (p.V == 4)
-        launch_mode<T, (sizeof(T) == 4 ? 4 : 2)>(c, mode, hv, cur, nxt, ldc, j0, nvalid, p);
-      else if (p.V == 2)
-        launch_mode<T, 2>(c, mode, hv, cur, nxt, ldc, j0, nvalid, p);
-      else
-        launch_mode<T, 1>(c, mode, hv, cur, nxt, ldc, j0, nvalid, p);
+      launch_step<T>(c, mode, hv, cur, nxt, ldc, j0, nvalid, pc);
       if (c->timing) LTB_HIP(c, hipEventRecord(c->events[2 * launches + 1], c->stream));
       LTB_HIP(c, hipGetLastError());
       const int streams = mode == 0 ? 1 : (mode == 1 ? 2 : 3);
       bytes += (double)streams * sizeof(T) * c->norb * (double)nvalid + hbytes;
       launches++;
-      hipLaunchKernelGGL(k_cheb_reduce, dim3(ldc / 64), dim3(256), 0, c->stream, c->d_part, p.n_sg,
-                         ldc, nvalid, j0, n, M, c->d_mu1, c->d_mom, c->nat);
+      hipLaunchKernelGGL(k_cheb_reduce, dim3(ldc / 64), dim3(256), 0, c->stream, c->d_part,
+                         sweep ? pc.slots : pc.n_sg, sweep ? pc.W : 0, pc.slots, ldc, nvalid, j0, n,
+                         M, c->d_mu1, c->d_mom, c->nat);
       LTB_HIP(c, hipGetLastError());
     }
   }
@@ -361,9 +475,57 @@ int run_moments(ltb_ctx* c, int M) {
   return LTB_OK;
 }
 
+// time `steps` general recursion steps (MODE 2) of one full-width chunk
+template <typename T>
+int time_steps(ltb_ctx* c, int steps, double* seconds, double* bytes) {
+  const Plan p = make_plan<T>(c);
+  int st = ensure_buffers<T>(c, p, -1);
+  if (st) return st;
+  const T* hv = sizeof(T) == 4 ? (const T*)c->d_ht32 : (const T*)c->d_ht64;
+  const int nvalid = std::min(p.ld, c->norb);
+  const int ldc = (nvalid + p.W - 1) / p.W * p.W;
+  LTB_HIP(c, hipMemsetAsync(c->d_xa, 0, (size_t)c->norb * ldc * sizeof(T), c->stream));
+  LTB_HIP(c, hipMemsetAsync(c->d_xb, 0, (size_t)c->norb * ldc * sizeof(T), c->stream));
+  hipEvent_t e0, e1;
+  LTB_HIP(c, hipEventCreate(&e0));
+  LTB_HIP(c, hipEventCreate(&e1));
+  launch_step<T>(c, 2, hv, (T*)c->d_xa, (T*)c->d_xb, ldc, 0, nvalid, p);  // warm
+  LTB_HIP(c, hipEventRecord(e0, c->stream));
+  for (int i = 0; i < steps; i++)
+    launch_step<T>(c, 2, hv, (i & 1) ? (T*)c->d_xb : (T*)c->d_xa, (i & 1) ? (T*)c->d_xa : (T*)c->d_xb,
+                   ldc, 0, nvalid, p);
+  LTB_HIP(c, hipEventRecord(e1, c->stream));
+  LTB_HIP(c, hipEventSynchronize(e1));
+  LTB_HIP(c, hipGetLastError());
+  float ms = 0;
+  LTB_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *seconds = ms * 1e-3;
+  const double hbytes = (double)c->nnzb * (16.0 * sizeof(T) + 4.0) + 4.0 * (c->nat + 1);
+  *bytes = steps * (3.0 * sizeof(T) * c->norb * (double)nvalid + hbytes);
+  return LTB_OK;
+}
+
 }  // namespace
 
 int kpe_moments_impl(ltb_ctx* c, int M, int precision) {
   if (precision == LTB_FP32_ACC64) return run_moments<float>(c, M);
   return run_moments<double>(c, M);
+}
+
+int kpe_time_steps(ltb_ctx* c, int precision, int steps, double* seconds, double* bytes) {
+  if (precision == LTB_FP32_ACC64) return time_steps<float>(c, steps, seconds, bytes);
+  return time_steps<double>(c, steps, seconds, bytes);
+}
+
+int kpe_resolved_tiling(ltb_ctx* c, int precision, ltb_kpe_tiling* out) {
+  Plan p = precision == LTB_FP32_ACC64 ? make_plan<float>(c) : make_plan<double>(c);
+  out->chunk_cols = p.ld;
+  out->cols_per_lane = p.V;
+  out->schedule = p.schedule;
+  out->blocks_per_cu = p.schedule == LTB_SCHED_SWEEP ? p.slots * 8 / std::max(c->cu_count, 8) : 0;
+  out->atoms_per_wave = p.schedule == LTB_SCHED_BLOCKED ? p.apw : 0;
+  out->nontemporal = p.nt;
+  return LTB_OK;
 }

@@ -52,9 +52,8 @@ struct ltb_ctx {
   // Fermi / density
   double *d_W = nullptr, *d_X = nullptr, *d_g = nullptr, *d_c = nullptr, *d_w = nullptr;
   int fd_K = 0, fd_M = 0;
-  // tiling
-  int chunk_cols = 0;       // 0 = auto
-  int atoms_per_wave = 0;   // 0 = auto
+  // tiling of the Chebyshev kernel (0 / -1 fields = auto)
+  ltb_kpe_tiling tiling = {0, 0, 0, 0, 0, -1};
   // statistics of the last ltb_kpe_moments
   int stat_launches = 0;
   double stat_seconds = 0, stat_bytes = 0;
@@ -77,5 +76,8 @@ int set_err(ltb_ctx* c, int code, const std::string& msg);
 // helpers implemented in ltb_capi.hip, used by sp2.hip
 int ensure_buffer(ltb_ctx* c, void** p, size_t* cap, size_t bytes);
 int kpe_moments_impl(ltb_ctx* c, int M, int precision);
+int kpe_time_steps(ltb_ctx* c, int precision, int steps, double* seconds, double* bytes);
+int kpe_resolved_tiling(ltb_ctx* c, int precision, ltb_kpe_tiling* out);
+typedef ltb_kpe_tiling KpeTiling;
 int sp2_dense_impl(ltb_ctx* c, double Ne, int max_iter, int fp64_tail, double* diagP, double* pop,
                    ltb_sp2_result* res);

@@ -753,10 +753,61 @@ int ltb_set_kpe_timing(ltb_ctx* c, int on) {
   return LTB_OK;
 }
 
-int ltb_set_kpe_tiling(ltb_ctx* c, int chunk_cols, int apw) {
-  if (!c || chunk_cols < 0 || apw < 0) return LTB_ERR_ARG;
-  c->chunk_cols = chunk_cols;
-  c->atoms_per_wave = apw;
+int ltb_set_kpe_tiling(ltb_ctx* c, const ltb_kpe_tiling* t) {
+  if (!c || !t) return LTB_ERR_ARG;
+  if (t->chunk_cols < 0 || t->atoms_per_wave < 0 || t->blocks_per_cu < 0 || t->schedule < 0 ||
+      t->schedule > 2 || (t->cols_per_lane != 0 && t->cols_per_lane != 1 && t->cols_per_lane != 2 &&
+                          t->cols_per_lane != 4))
+    return set_err(c, LTB_ERR_ARG, "bad tiling");
+  c->tiling = *t;
+  return LTB_OK;
+}
+
+int ltb_get_kpe_tiling(ltb_ctx* c, int precision, ltb_kpe_tiling* out) {
+  int st = need_system(c);
+  if (st) return st;
+  if (!out) return set_err(c, LTB_ERR_ARG, "null output");
+  return kpe_resolved_tiling(c, precision, out);
+}
+
+int ltb_kpe_autotune(ltb_ctx* c, int precision, int trial_steps, ltb_kpe_tiling* best,
+                     double* best_gbs) {
+  int st = need_system(c);
+  if (st) return st;
+  if (trial_steps < 1) trial_steps = 3;
+  const ltb_kpe_tiling saved = c->tiling;
+  std::vector<ltb_kpe_tiling> cand;
+  const int chunks[] = {2048, 4096, 8192};
+  const int vs[] = {1, 2, 4};
+  for (int ch : chunks)
+    for (int v : vs)
+      for (int nt = 0; nt <= 1; nt++) {
+        cand.push_back({ch, v, LTB_SCHED_BLOCKED, 0, 0, nt});
+        for (int bpc : {2, 4, 8}) cand.push_back({ch, v, LTB_SCHED_SWEEP, bpc, 0, nt});
+      }
+  double best_rate = -1;
+  ltb_kpe_tiling win = saved;
+  for (const ltb_kpe_tiling& t : cand) {
+    c->tiling = t;
+    ltb_kpe_tiling r;
+    kpe_resolved_tiling(c, precision, &r);
+    if (r.schedule != t.schedule) continue;  // sweep needs >= 8 column groups
+    if (t.chunk_cols > 4 * c->norb) continue;
+    double secs = 0, bytes = 0;
+    st = kpe_time_steps(c, precision, trial_steps, &secs, &bytes);
+    if (st) {
+      c->tiling = saved;
+      return st;
+    }
+    double rate = bytes / secs;
+    if (rate > best_rate) {
+      best_rate = rate;
+      win = t;
+    }
+  }
+  c->tiling = win;
+  if (best) kpe_resolved_tiling(c, precision, best);
+  if (best_gbs) *best_gbs = best_rate / 1e9;
   return LTB_OK;
 }
 

@@ -73,17 +73,58 @@ def test_ragged_chunks_equal_single_chunk(ctx, prec):
     s = L.System.quartz(3, 3, 3, sigma_ang=0.1, seed=2)  # 972 orbitals
     prepare(ctx, s)
     try:
-        ctx.set_tiling(0, 0)
+        ctx.set_tiling()
         ref, _ = ctx.kpe_moments(64, prec)
-        ctx.set_tiling(128, 1)  # 8 chunks, the last one 76 columns wide
-        got, _ = ctx.kpe_moments(64, prec)
-        ctx.set_tiling(320, 3)
+        ctx.set_tiling(chunk_cols=128, cols_per_lane=1, schedule=L.SCHED_BLOCKED, atoms_per_wave=1)
+        got, _ = ctx.kpe_moments(64, prec)  # 8 chunks, the last one 76 columns wide
+        ctx.set_tiling(chunk_cols=320, cols_per_lane=1, schedule=L.SCHED_BLOCKED, atoms_per_wave=3)
         got2, _ = ctx.kpe_moments(64, prec)
     finally:
-        ctx.set_tiling(0, 0)
+        ctx.set_tiling()
     tol = 1e-12 if prec == L.FP64 else 2e-6
     assert np.abs(got - ref).max() < tol
     assert np.abs(got2 - ref).max() < tol
+
+
+TILINGS = [dict(schedule=sch, cols_per_lane=v, nontemporal=nt, chunk_cols=ch, blocks_per_cu=bpc)
+           for sch, v, nt, ch, bpc in [
+               (L.SCHED_BLOCKED, 1, 0, 4096, 0), (L.SCHED_BLOCKED, 2, 1, 2048, 0),
+               (L.SCHED_BLOCKED, 4, 1, 4096, 0), (L.SCHED_SWEEP, 1, 1, 4096, 4),
+               (L.SCHED_SWEEP, 2, 0, 4096, 2), (L.SCHED_SWEEP, 4, 1, 8192, 8),
+               (L.SCHED_SWEEP, 1, 1, 5000, 3)]]
+
+
+@pytest.mark.parametrize("prec", [L.FP64, L.FP32_ACC64])
+def test_every_tiling_gives_the_same_moments(ctx, prec):
+    """18,432 orbitals: both schedules, V in {1,2,4}, cached / non-temporal,
+    chunk widths that leave a ragged last chunk (5000 -> 3 x 5056 + 3264)."""
+    s = L.System.quartz(8, 8, 8, sigma_ang=0.05, seed=1)
+    prepare(ctx, s)
+    try:
+        ctx.set_tiling(schedule=L.SCHED_BLOCKED, cols_per_lane=1, chunk_cols=4096)
+        ref, _ = ctx.kpe_moments(24, L.FP64)
+        for t in TILINGS:
+            ctx.set_tiling(**t)
+            got, _ = ctx.kpe_moments(24, prec)
+            tol = 1e-12 if prec == L.FP64 else 3e-6
+            assert np.abs(got - ref).max() < tol, t
+    finally:
+        ctx.set_tiling()
+
+
+def test_autotune_picks_a_valid_tiling(ctx):
+    s = L.System.quartz(8, 8, 8, sigma_ang=0.05, seed=1)
+    prepare(ctx, s)
+    try:
+        best, gbs = ctx.autotune(L.FP32_ACC64, trial_steps=2)
+        assert gbs > 100
+        assert best["cols_per_lane"] in (1, 2, 4) and best["schedule"] in (1, 2)
+        am, _ = ctx.kpe_moments(24, L.FP32_ACC64)
+        ctx.set_tiling(schedule=L.SCHED_BLOCKED, cols_per_lane=1)
+        ref, _ = ctx.kpe_moments(24, L.FP64)
+        assert np.abs(am - ref).max() < 3e-6
+    finally:
+        ctx.set_tiling()
 
 
 def test_density_matches_oracle(ctx, c0):
