@@ -134,7 +134,8 @@ typedef struct {
   double tol;         /* |q_out - q_in|_inf in e (config 1e-6)                  */
   int max_iter;       /* cap (reference: 20, :1820)                             */
   int precision;      /* enum ltb_precision for the bulk iterations            */
-  double switch_tol;  /* FP32 bulk -> FP64 once |dq| < switch_tol (0: never)   */
+  double switch_tol;  /* FP32 bulk -> FP64 once |dq| < switch_tol; convergence is
+                         then declared on an FP64 iteration only (0: FP32 to the end) */
   int use_graph;      /* capture the device part of an iteration in a hipGraph */
 } ltb_scc_params;
 
@@ -143,7 +144,10 @@ typedef struct {
   int fp64_iterations;
   double dq;          /* last |q_out - q_in|_inf (e)                            */
   double mu_ha;
-  double eband;       /* Hartree, at the last iteration                          */
+  double eband;       /* Hartree, 2 Tr(P H) of the last iteration (H from q_in)  */
+  double eband_fp;    /* Hartree, eband + 2 pop.Gamma(p_out - p_in): E_band at the
+                         fixed point to first order in |q_out - q*|             */
+  int graph_launches; /* iterations replayed from a captured hipGraph            */
   double seconds;     /* wall time of the loop                                   */
 } ltb_scc_result;
 

@@ -69,7 +69,8 @@ class SccParams(C.Structure):
 
 class SccResult(C.Structure):
     _fields_ = [("iterations", C.c_int), ("fp64_iterations", C.c_int), ("dq", C.c_double),
-                ("mu_ha", C.c_double), ("eband", C.c_double), ("seconds", C.c_double)]
+                ("mu_ha", C.c_double), ("eband", C.c_double), ("eband_fp", C.c_double),
+                ("graph_launches", C.c_int), ("seconds", C.c_double)]
 
 
 class Sp2Result(C.Structure):
@@ -291,7 +292,8 @@ class Context:
         if check and st not in (LTB_OK,):
             self._check(st)
         return dict(status=st, pop=pout, q=2 * pout, iters=r.iterations, fp64_iters=r.fp64_iterations,
-                    dq=r.dq, mu_ha=r.mu_ha, eband=r.eband, seconds=r.seconds,
+                    dq=r.dq, mu_ha=r.mu_ha, eband=r.eband, eband_fp=r.eband_fp,
+                    graph_launches=r.graph_launches, seconds=r.seconds,
                     hist=hist[:r.iterations])
 
     def kpe_apply(self, M, w, X, precision=FP64):

@@ -15,6 +15,7 @@ struct DevScalars {
   double nelec;        // 2 * sum w_m mu_m
   double eband;        // 2 Tr(P H_total)
   double dq;           // |q_out - q_in|_inf (e)
+  double eband_fp;     // eband + 2 pop . Gamma (p_out - p_in): fixed-point estimate
   int nonfinite;       // set by reductions that met NaN/Inf
   int pad;
 };

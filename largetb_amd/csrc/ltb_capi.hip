@@ -230,43 +230,60 @@ __global__ void k_eband(const double* __restrict__ tot, const double* __restrict
   if (!isfinite(sc->eband) || !isfinite(sc->mu)) sc->nonfinite = 1;
 }
 
-// |q_out - q_in|_inf (e) and linear mixing p_in += mix (p_out - p_in)
+// |q_out - q_in|_inf (e), and the first-order band-energy correction
+// 2 sum_A pop_A [Gamma (p_out - p_in)]_A: E_band depends on p_in through the
+// Coulomb shift with gradient 2 Gamma pop (Hellmann-Feynman, gap >> kT), so
+// E_band(p_in) + 2 pop . Gamma (p_out - p_in) estimates E_band at the fixed
+// point to O(|p_out - p*|).
 __global__ __launch_bounds__(256) void k_dq_partial(const double* __restrict__ pin,
-                                                    const double* __restrict__ pout, int nat,
+                                                    const double* __restrict__ pout,
+                                                    const int* __restrict__ grow,
+                                                    const int* __restrict__ gcol,
+                                                    const double* __restrict__ gval, int nat,
                                                     double* __restrict__ red) {
-  double mx = 0;
+  double mx = 0, corr = 0;
   int bad = 0;
   for (int a = blockIdx.x * 256 + threadIdx.x; a < nat; a += gridDim.x * 256) {
     double d = 2.0 * fabs(pout[a] - pin[a]);
     if (!isfinite(d)) bad = 1;
     mx = fmax(mx, d);
+    double gd = 0;
+    for (int k = grow[a]; k < grow[a + 1]; k++) gd += gval[k] * (pout[gcol[k]] - pin[gcol[k]]);
+    corr += pout[a] * gd;
   }
-  __shared__ double sh[256];
+  __shared__ double sh[256], sc2[256];
   __shared__ int sb;
   if (threadIdx.x == 0) sb = 0;
   __syncthreads();
   if (bad) atomicOr(&sb, 1);
   sh[threadIdx.x] = mx;
+  sc2[threadIdx.x] = corr;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+    if ((int)threadIdx.x < w) {
+      sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+      sc2[threadIdx.x] += sc2[threadIdx.x + w];
+    }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    red[2 * blockIdx.x] = sh[0];
-    red[2 * blockIdx.x + 1] = sb ? 1.0 : 0.0;
+    red[3 * blockIdx.x] = sh[0];
+    red[3 * blockIdx.x + 1] = sb ? 1.0 : 0.0;
+    red[3 * blockIdx.x + 2] = sc2[0];
   }
 }
 
 __global__ void k_dq_final(const double* __restrict__ red, int nb, DevScalars* sc) {
   if (threadIdx.x != 0) return;
-  double mx = 0;
+  double mx = 0, corr = 0;
   int bad = 0;
   for (int b = 0; b < nb; b++) {
-    mx = fmax(mx, red[2 * b]);
-    if (red[2 * b + 1] != 0.0) bad = 1;
+    mx = fmax(mx, red[3 * b]);
+    if (red[3 * b + 1] != 0.0) bad = 1;
+    corr += red[3 * b + 2];
   }
   sc->dq = mx;
+  sc->eband_fp = sc->eband + 2.0 * corr;
   sc->nonfinite |= bad;
 }
 
@@ -371,8 +388,8 @@ int rescale_async(ltb_ctx* c) {
 
 int mix_async(ltb_ctx* c, double mix) {
   int nb = std::min(kRedBlocks, grid_for(c->nat, 256));
-  hipLaunchKernelGGL(k_dq_partial, dim3(nb), dim3(256), 0, c->stream, c->d_pin, c->d_pout, c->nat,
-                     c->d_red);
+  hipLaunchKernelGGL(k_dq_partial, dim3(nb), dim3(256), 0, c->stream, c->d_pin, c->d_pout,
+                     c->d_grow, c->d_gcol, c->d_gval, c->nat, c->d_red);
   hipLaunchKernelGGL(k_dq_final, dim3(1), dim3(64), 0, c->stream, c->d_red, nb, c->d_sc);
   hipLaunchKernelGGL(k_mix, dim3(grid_for(c->nat, 256)), dim3(256), 0, c->stream, c->d_pin,
                      c->d_pout, mix, c->nat);
@@ -689,11 +706,24 @@ int ltb_density(ltb_ctx* c, const ltb_fermi_params* fp, double* pop, ltb_density
   return LTB_OK;
 }
 
+// one SCC iteration, device only: constructdeltaH, Gershgorin, rescale,
+// exact-trace moments, mu bisection + populations + band energy, dq + mixing
+int scc_iteration_async(ltb_ctx* c, const ltb_scc_params* prm, const ltb_fermi_params* fp, int prec) {
+  int s2;
+  hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(64), 0, c->stream, c->d_sc);
+  if ((s2 = shift_async(c)) || (s2 = gersh_async(c)) || (s2 = rescale_async(c))) return s2;
+  if ((s2 = kpe_moments_impl(c, prm->M, prec))) return s2;
+  if ((s2 = density_async(c, fp))) return s2;
+  return mix_async(c, prm->mix);
+}
+
 int ltb_scc_run(ltb_ctx* c, const ltb_scc_params* prm, const double* guess, double* p_out,
                 double* hist, ltb_scc_result* res) {
   int st = need_system(c);
   if (st) return st;
-  if (!prm || prm->M < 2 || prm->max_iter < 1 || !(prm->tol > 0) || !(prm->mix > 0) || prm->mix > 1)
+  if (!prm || prm->M < 2 || prm->max_iter < 1 || !(prm->tol > 0) || !(prm->mix > 0) || prm->mix > 1 ||
+      (prm->kT > 0 && prm->K < 16) ||
+      (prm->precision != LTB_FP64 && prm->precision != LTB_FP32_ACC64))
     return set_err(c, LTB_ERR_ARG, "bad SCC parameters");
   auto t0 = std::chrono::steady_clock::now();
   if (guess)
@@ -702,28 +732,54 @@ int ltb_scc_run(ltb_ctx* c, const ltb_scc_params* prm, const double* guess, doub
     LTB_HIP(c, hipMemcpyAsync(c->d_pin, c->d_p0, sizeof(double) * c->nat, hipMemcpyDeviceToDevice, c->stream));
   ltb_fermi_params fp{prm->M, prm->kT, prm->K, c->nelectrons};
   int prec = prm->precision;
-  int it = 0, n64 = 0;
+  int it = 0, n64 = 0, ngraph = 0;
   DevScalars h{};
+  const bool saved_timing = c->timing;
+  hipGraphExec_t exec[2] = {nullptr, nullptr};  // [FP64, FP32]
+  bool warmed[2] = {false, false};
   st = LTB_NOT_CONVERGED;
   for (it = 0; it < prm->max_iter; it++) {
-    hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(64), 0, c->stream, c->d_sc);
-    int s2;
-    if ((s2 = shift_async(c)) || (s2 = gersh_async(c)) || (s2 = rescale_async(c))) return s2;
-    if ((s2 = kpe_moments_impl(c, prm->M, prec))) return s2;
+    const int gi = prec == LTB_FP64 ? 0 : 1;
+    int s2 = LTB_OK;
+    if (prm->use_graph && warmed[gi]) {
+      if (!exec[gi]) {
+        // capture once; buffers were sized by the eager warm iteration
+        c->timing = false;
+        hipGraph_t g = nullptr;
+        LTB_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        s2 = scc_iteration_async(c, prm, &fp, prec);
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (s2) return s2;
+        if (e != hipSuccess) return set_err(c, LTB_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&exec[gi], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return set_err(c, LTB_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+      }
+      LTB_HIP(c, hipGraphLaunch(exec[gi], c->stream));
+      ngraph++;
+    } else {
+      if ((s2 = scc_iteration_async(c, prm, &fp, prec))) return s2;
+      warmed[gi] = true;
+    }
     if (prec == LTB_FP64) n64++;
-    if ((s2 = density_async(c, &fp))) return s2;
-    // populations -> pout; dq and mixing (q_out kept in pout)
-    if ((s2 = mix_async(c, prm->mix))) return s2;
     if ((s2 = read_scalars(c, &h))) return s2;
-    if (h.nonfinite) return set_err(c, LTB_ERR_NONFINITE, "non-finite value in SCC iteration " + std::to_string(it));
+    if (h.nonfinite) {
+      st = set_err(c, LTB_ERR_NONFINITE, "non-finite value in SCC iteration " + std::to_string(it));
+      it++;
+      break;
+    }
     if (hist) hist[it] = h.dq;
-    if (h.dq < prm->tol) {
+    const bool final_precision = prec == LTB_FP64 || prm->switch_tol <= 0;
+    if (h.dq < prm->tol && final_precision) {
       st = LTB_OK;
       it++;
       break;
     }
     if (prec != LTB_FP64 && prm->switch_tol > 0 && h.dq < prm->switch_tol) prec = LTB_FP64;
   }
+  c->timing = saved_timing;
+  for (hipGraphExec_t e : exec)
+    if (e) (void)hipGraphExecDestroy(e);
   if (p_out) LTB_HIP(c, hipMemcpy(p_out, c->d_pout, sizeof(double) * c->nat, hipMemcpyDeviceToHost));
   auto t1 = std::chrono::steady_clock::now();
   if (res) {
@@ -732,6 +788,8 @@ int ltb_scc_run(ltb_ctx* c, const ltb_scc_params* prm, const double* guess, doub
     res->dq = h.dq;
     res->mu_ha = h.a * h.mu + h.b;
     res->eband = h.eband;
+    res->eband_fp = h.eband_fp;
+    res->graph_launches = ngraph;
     res->seconds = std::chrono::duration<double>(t1 - t0).count();
   }
   if (st == LTB_NOT_CONVERGED)

@@ -175,14 +175,50 @@ def test_scc_fp64_matches_oracle(ctx, c0):
     assert 2 * g["pop"].sum() == pytest.approx(384.0, abs=1e-8)
 
 
-def test_scc_mixed_precision_matches_fp64(ctx, c0):
+EV = L.HARTREE_EV
+
+
+@pytest.fixture(scope="module")
+def c0_fixed_point(c0):
+    """Oracle SCC converged to 1e-12 e and E_band evaluated at q*."""
+    _, o = c0
+    r = o.scc(200, tol=1e-12, maxit=400)
+    v = o.dense_gamma() @ (r["pop"] - o.p0)
+    d = o.density(200, shift=np.repeat(v, 4))
+    return r["q"], d["eband"]
+
+
+def test_scc_mixed_precision_matches_fp64(ctx, c0, c0_fixed_point):
+    """FP32 bulk, FP64 iterations once |dq| < 1e-6 until an FP64 iteration
+    converges: charges within 1e-6 e and E_band (fixed-point estimate) within
+    1e-6 eV/atom of the plain FP64 solve."""
     s, o = c0
+    qstar, estar = c0_fixed_point
     ctx.load(s)
     ref = ctx.scc(200, precision=L.FP64, tol=1e-6, max_iter=300)
-    mix = ctx.scc(200, precision=L.FP32_ACC64, switch_tol=1e-4, tol=1e-6, max_iter=300)
-    assert mix["status"] == 0
-    assert mix["fp64_iters"] >= 1
+    mix = ctx.scc(200, precision=L.FP32_ACC64, switch_tol=1e-6, tol=1e-6, max_iter=300)
+    assert mix["status"] == 0 and 1 <= mix["fp64_iters"] <= 5
     assert np.abs(mix["q"] - ref["q"]).max() < 1e-6
+    assert abs(mix["eband_fp"] - ref["eband_fp"]) * EV / s.nat < 1e-6
+    # both against the tightly converged oracle fixed point
+    assert np.abs(ref["q"] - qstar).max() < 1e-7
+    assert abs(ref["eband_fp"] - estar) * EV / s.nat < 2e-6
+    # the uncorrected last-iteration value is first order in |q_in - q*|
+    assert abs(ref["eband"] - estar) * EV / s.nat > 1e-5
+
+
+def test_scc_graph_replays_match_eager(ctx, c0):
+    """hipGraph-captured iterations (replayed many times, host reads between
+    replays) reproduce the eager loop bit for bit."""
+    s, _ = c0
+    ctx.load(s)
+    for prec, sw in [(L.FP64, 0.0), (L.FP32_ACC64, 1e-4)]:
+        eager = ctx.scc(200, precision=prec, switch_tol=sw, tol=1e-6, max_iter=300, use_graph=0)
+        graph = ctx.scc(200, precision=prec, switch_tol=sw, tol=1e-6, max_iter=300, use_graph=1)
+        assert graph["graph_launches"] >= graph["iters"] - 3
+        assert graph["iters"] == eager["iters"]
+        assert np.array_equal(graph["q"], eager["q"])
+        assert np.array_equal(graph["hist"], eager["hist"])
 
 
 def test_scc_iteration_cap_reports_not_converged(ctx, c0):
