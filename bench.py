@@ -62,6 +62,39 @@ def cpu_baseline(cells, sigma, seed, M, target_s=15.0):
             "scc_iter_s_extrapolated": round(iter_s, 1)}
 
 
+def measured_traffic(cells, M, tiling, prec_name):
+    """HBM bytes per SpMM launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE / WRITE_SIZE scaled by a plain-copy calibration), when this
+    run's workload and tiling are the profiled ones; otherwise None."""
+    import re
+    path = os.path.join(ROOT, "profiles", "cheb_sweep_fp32_quartz12_M512.txt")
+    if not (cells == [12, 12, 12] and prec_name == "fp32" and tiling and os.path.exists(path)):
+        return None
+    want = (4096, 4, 2, 2, 1)
+    got = tuple(tiling[k] for k in ("chunk_cols", "cols_per_lane", "schedule", "blocks_per_cu", "nontemporal"))
+    if got != want:
+        return None
+    m = re.search(r"= ([0-9.]+) GB per launch", open(path).read())
+    return float(m.group(1)) * 1e9 if m else None
+
+
+def copy_calibration(nbytes=1 << 30, reps=3):
+    """Plain device-to-device copies of a known size: the reference point that
+    FETCH_SIZE / WRITE_SIZE readings of the SpMM kernel are scaled by."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,7 +107,14 @@ def main():
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tiling", type=str, default="",
+                    help="explicit tiling (implies --no-autotune): chunk,V,schedule,blocks_per_cu,nt")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 1 GiB device-to-device copies (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU sharding is design-only "
               "(column blocks per GPU + an all-reduce of the moments), so --gpus must be 1",
@@ -88,7 +128,11 @@ def main():
     ctx = L.Context(0)
     ctx.load(s)
     tiling = None
-    if not args.no_autotune:
+    if args.tiling:
+        ch, v, sch, bpc, nt = (int(x) for x in args.tiling.split(","))
+        ctx.set_tiling(chunk_cols=ch, cols_per_lane=v, schedule=sch, blocks_per_cu=bpc, nontemporal=nt)
+        tiling = ctx.get_tiling(prec)
+    elif not args.no_autotune:
         ctx.set_populations(None)
         lo, hi = ctx.gershgorin()
         ctx.rescale((hi - lo) / 2, (hi + lo) / 2)
@@ -122,7 +166,8 @@ def main():
                    "atoms": s.nat, "orbitals": s.norb, "M": args.M, "sigma_ang": args.sigma,
                    "seed": args.seed},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(args.cells, args.M, tiling, args.precision),
                      "kernel": "k_cheb_step", "launches": st["launches"],
                      "mean_launch_us": round(st["seconds"] / max(st["launches"], 1) * 1e6, 2),
                      "tiling": tiling},

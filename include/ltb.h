@@ -168,7 +168,9 @@ int ltb_kpe_apply(ltb_ctx* ctx, int M, const double* w, int ncols, const double*
  * matrix: X0 = (emax I - H_total)/(emax - emin) from the Gershgorin bounds,
  * X <- X^2 if 2 tr X > Ne else 2X - X^2, FP32 MFMA GEMMs until the
  * idempotency error stops falling (or max_iter), then fp64_tail FP64 MFMA
- * iterations.  diagP (optional, [norb]) and pop (optional, per-spin [natoms])
+ * iterations.  The FP64 tail restores idempotency and the trace; the occupied
+ * subspace keeps the FP32 rounding (~1e-6 per diagonal element).  max_iter = 0
+ * runs FP64 from the start, with fp64_tail as its cap.  diagP (optional, [norb]) and pop (optional, per-spin [natoms])
  * receive diag(X) and its per-atom sums. */
 typedef struct {
   int iterations, fp64_iterations;
