@@ -137,6 +137,7 @@ typedef struct {
   double switch_tol;  /* FP32 bulk -> FP64 once |dq| < switch_tol; convergence is
                          then declared on an FP64 iteration only (0: FP32 to the end) */
   int use_graph;      /* capture the device part of an iteration in a hipGraph */
+  int verbose;        /* 1: one line per iteration on stderr                     */
 } ltb_scc_params;
 
 typedef struct {
@@ -214,6 +215,31 @@ int ltb_get_kpe_tiling(ltb_ctx* ctx, int precision, ltb_kpe_tiling* out);
  * algorithmic GB/s. */
 int ltb_kpe_autotune(ltb_ctx* ctx, int precision, int trial_steps, ltb_kpe_tiling* best,
                      double* best_gbs);
+
+/* ---- learned SCC starting guess ----------------------------------------------
+ * No reference counterpart: the reference starts every SCC from the neutral
+ * charges (arrangeCoordinates :714-720).  A per-species MLP on local
+ * descriptors predicts converged charges; the plain neutral start remains the
+ * fallback and the judge of the answer. */
+#define LTB_MLP_NFEAT 80 /* 2 x 32 radial Gaussians (per neighbour species) + 16 angular */
+typedef struct ltb_mlp ltb_mlp;
+int ltb_mlp_num_params(void);
+int ltb_mlp_create(ltb_ctx* ctx, unsigned long long seed, ltb_mlp** out);
+void ltb_mlp_destroy(ltb_mlp* mlp);
+int ltb_mlp_get_params(ltb_mlp* mlp, float* params);
+int ltb_mlp_set_params(ltb_mlp* mlp, const float* params); /* also resets the Adam state */
+/* descriptors of every atom of sys (cutoff 6 A, periodic images), [natoms][80] */
+int ltb_descriptors(ltb_ctx* ctx, const ltb_system* sys, float* feat);
+/* training / evaluation set: atoms grouped in nseg structures (seg[0..nseg]),
+ * qtot[s] = electrons of structure s, target = charges in e (or NULL) */
+int ltb_mlp_set_data(ltb_mlp* mlp, int natoms, const float* feat, const int* Z, int nseg,
+                     const int* seg, const double* qtot, const double* target);
+int ltb_mlp_fit_normalization(ltb_mlp* mlp); /* feature mean/sd and charge shift per species */
+int ltb_mlp_forward(ltb_mlp* mlp, double* q);  /* charges (e), neutral per structure */
+int ltb_mlp_loss_grad(ltb_mlp* mlp, double* loss, float* grad /* [trainable] or NULL */);
+int ltb_mlp_adam(ltb_mlp* mlp, int steps, float lr, double* loss_hist /* [steps] or NULL */);
+/* per-spin population guess for ltb_scc_run (replaces the data set) */
+int ltb_mlp_predict(ltb_mlp* mlp, const ltb_system* sys, double* p_guess);
 
 #ifdef __cplusplus
 }

@@ -64,7 +64,7 @@ SCHED_AUTO, SCHED_BLOCKED, SCHED_SWEEP = 0, 1, 2
 class SccParams(C.Structure):
     _fields_ = [("M", C.c_int), ("kT", C.c_double), ("K", C.c_int), ("mix", C.c_double),
                 ("tol", C.c_double), ("max_iter", C.c_int), ("precision", C.c_int),
-                ("switch_tol", C.c_double), ("use_graph", C.c_int)]
+                ("switch_tol", C.c_double), ("use_graph", C.c_int), ("verbose", C.c_int)]
 
 
 class SccResult(C.Structure):
@@ -111,7 +111,24 @@ SIGNATURES = {
     "ltb_set_kpe_tiling": (C.c_int, [VP, C.POINTER(KpeTiling)]),
     "ltb_get_kpe_tiling": (C.c_int, [VP, C.c_int, C.POINTER(KpeTiling)]),
     "ltb_kpe_autotune": (C.c_int, [VP, C.c_int, C.c_int, C.POINTER(KpeTiling), PD]),
+    "ltb_mlp_num_params": (C.c_int, []),
+    "ltb_mlp_create": (C.c_int, [VP, C.c_ulonglong, C.POINTER(VP)]),
+    "ltb_mlp_destroy": (None, [VP]),
+    "ltb_mlp_get_params": (C.c_int, [VP, C.POINTER(C.c_float)]),
+    "ltb_mlp_set_params": (C.c_int, [VP, C.POINTER(C.c_float)]),
+    "ltb_descriptors": (C.c_int, [VP, VP, C.POINTER(C.c_float)]),
+    "ltb_mlp_set_data": (C.c_int, [VP, C.c_int, C.POINTER(C.c_float), PI, C.c_int, PI, PD, PD]),
+    "ltb_mlp_fit_normalization": (C.c_int, [VP]),
+    "ltb_mlp_forward": (C.c_int, [VP, PD]),
+    "ltb_mlp_loss_grad": (C.c_int, [VP, PD, C.POINTER(C.c_float)]),
+    "ltb_mlp_adam": (C.c_int, [VP, C.c_int, C.c_float, PD]),
+    "ltb_mlp_predict": (C.c_int, [VP, VP, PD]),
 }
+
+MLP_NFEAT = 80
+MLP_HIDDEN = 64
+MLP_PER_SPECIES = MLP_HIDDEN * MLP_NFEAT + MLP_HIDDEN + 2 * (MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN) + MLP_HIDDEN + 1
+MLP_TRAINABLE = 2 * MLP_PER_SPECIES
 
 
 def lib():
@@ -281,9 +298,9 @@ class Context:
         return dict(pop=pop, mu=r.mu, mu_ha=r.mu_ha, a=r.a, b=r.b, nelec=r.nelec, eband=r.eband)
 
     def scc(self, M, guess=None, kT_ev=0.05, K=8192, mix=0.2, tol=1e-6, max_iter=200,
-            precision=FP64, switch_tol=0.0, use_graph=0, check=True):
+            precision=FP64, switch_tol=0.0, use_graph=0, check=True, verbose=0):
         kT = kT_ev / HARTREE_EV if kT_ev > 0 else 0.0
-        prm = SccParams(M, kT, K, mix, tol, max_iter, precision, switch_tol, use_graph)
+        prm = SccParams(M, kT, K, mix, tol, max_iter, precision, switch_tol, use_graph, verbose)
         pout = np.zeros(self.nat)
         hist = np.zeros(max_iter)
         r = SccResult()
@@ -337,3 +354,76 @@ class Context:
         g = C.c_double()
         self._check(lib().ltb_kpe_autotune(self._h, precision, trial_steps, C.byref(t), C.byref(g)))
         return t.as_dict(), g.value
+
+    def descriptors(self, system):
+        f = np.zeros((system.nat, MLP_NFEAT), np.float32)
+        self._check(lib().ltb_descriptors(self._h, system._h, f.ctypes.data_as(C.POINTER(C.c_float))))
+        return f
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Mlp:
+    """Per-species 80-64-64-64-1 SiLU network on the device (FP32) with Adam."""
+
+    def __init__(self, ctx, seed=1):
+        self.ctx = ctx
+        h = VP()
+        ctx._check(lib().ltb_mlp_create(ctx._h, seed, C.byref(h)))
+        self._h = h
+        self.nparams = lib().ltb_mlp_num_params()
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            lib().ltb_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()
+
+    def params(self):
+        p = np.zeros(self.nparams, np.float32)
+        self.ctx._check(lib().ltb_mlp_get_params(self._h, _f(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.nparams
+        self.ctx._check(lib().ltb_mlp_set_params(self._h, _f(p)))
+
+    def set_data(self, feat, Z, seg, qtot, target=None):
+        feat = np.ascontiguousarray(feat, np.float32)
+        Z = np.ascontiguousarray(Z, np.int32)
+        seg = np.ascontiguousarray(seg, np.int32)
+        qtot = np.ascontiguousarray(qtot, np.float64)
+        self._target = None if target is None else np.ascontiguousarray(target, np.float64)
+        self.n = len(Z)
+        self.ctx._check(lib().ltb_mlp_set_data(self._h, len(Z), _f(feat), _i(Z), len(seg) - 1, _i(seg),
+                                               _d(qtot), _d(self._target) if self._target is not None else None))
+
+    def fit_normalization(self):
+        self.ctx._check(lib().ltb_mlp_fit_normalization(self._h))
+
+    def forward(self):
+        q = np.zeros(self.n)
+        self.ctx._check(lib().ltb_mlp_forward(self._h, _d(q)))
+        return q
+
+    def loss_grad(self):
+        loss = C.c_double()
+        g = np.zeros(MLP_TRAINABLE, np.float32)
+        self.ctx._check(lib().ltb_mlp_loss_grad(self._h, C.byref(loss), _f(g)))
+        return loss.value, g
+
+    def adam(self, steps, lr=1e-3):
+        hist = np.zeros(steps)
+        self.ctx._check(lib().ltb_mlp_adam(self._h, steps, lr, _d(hist)))
+        return hist
+
+    def predict(self, system):
+        p = np.zeros(system.nat)
+        self.ctx._check(lib().ltb_mlp_predict(self._h, system._h, _d(p)))
+        return p

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cstdio>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -769,6 +770,12 @@ int ltb_scc_run(ltb_ctx* c, const ltb_scc_params* prm, const double* guess, doub
       break;
     }
     if (hist) hist[it] = h.dq;
+    if (prm->verbose) {
+      double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      fprintf(stderr, "scc iter %d %s dq=%.3e mu=%.6f Ha eband_fp=%.10f Ha t=%.2fs\n", it,
+              prec == LTB_FP64 ? "fp64" : "fp32", h.dq, h.a * h.mu + h.b, h.eband_fp, el);
+      fflush(stderr);
+    }
     const bool final_precision = prec == LTB_FP64 || prm->switch_tol <= 0;
     if (h.dq < prm->tol && final_precision) {
       st = LTB_OK;

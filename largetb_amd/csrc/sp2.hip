@@ -257,7 +257,8 @@ int sp2_dense_impl(ltb_ctx* c, double Ne, int max_iter, int fp64_tail, double* d
       *idem_out = idem;
       if (!std::isfinite(idem)) return set_err(c, LTB_ERR_NONFINITE, "non-finite trace in SP2");
       // stop when the idempotency error no longer falls (rounding floor)
-      if (stall_stop && it >= 4 && idem < 1e-2 * norb && idem >= idem_hist[0]) {
+      // (quadratic regime reached and no halving over two steps)
+      if (stall_stop && it >= 4 && idem < 1e-4 * norb && idem >= 0.5 * idem_hist[0]) {
         it++;
         break;
       }

@@ -392,4 +392,112 @@ Status build_from_coords(ltb_system& s) {
   }
   return Status::ok();
 }
+Status pairs_within(const ltb_system& s, double rc, std::vector<int>& rowptr,
+                    std::vector<int>& nbj, std::vector<double>& nbd) {
+  const int n = s.natoms;
+  double inv[9];
+  const bool have_inv = invert3(s.cell, inv);
+  if (s.periodic_mask && !have_inv) return fail(1, "singular periodic cell");
+  const double rc2 = rc * rc;
+  rowptr.assign(n + 1, 0);
+  nbj.clear();
+  nbd.clear();
+  int reach[3] = {0, 0, 0};
+  bool minimum_image = have_inv;
+  for (int d = 0; d < 3; d++)
+    if (s.periodic_mask >> d & 1) {
+      double w = width(s.cell, d);
+      reach[d] = (int)std::ceil(rc / w) + 1;  // positions may sit anywhere in the cell
+      if (w <= 2.0 * rc) minimum_image = false;
+    }
+  if (minimum_image && n > 2000) {
+    // binned search, one image per pair (every periodic width exceeds 2 rc)
+    std::vector<double> f(3 * (size_t)n);
+    for (int i = 0; i < n; i++)
+      for (int d = 0; d < 3; d++)
+        f[3 * i + d] = s.xyz[3 * i] * inv[d] + s.xyz[3 * i + 1] * inv[3 + d] + s.xyz[3 * i + 2] * inv[6 + d];
+    int nbin[3];
+    double fmin[3], span[3];
+    for (int d = 0; d < 3; d++) {
+      bool per = s.periodic_mask >> d & 1;
+      double lo = 1e300, hi = -1e300;
+      for (int i = 0; i < n; i++) lo = std::min(lo, f[3 * i + d]), hi = std::max(hi, f[3 * i + d]);
+      fmin[d] = per ? 0.0 : lo;
+      span[d] = per ? 1.0 : std::max(hi - lo, 1e-9);
+      nbin[d] = std::max(1, std::min(512, (int)std::floor(width(s.cell, d) * span[d] / rc)));
+    }
+    auto bin_of = [&](int i, int d) {
+      double v = (f[3 * i + d] - fmin[d]) / span[d];
+      if (s.periodic_mask >> d & 1) v -= std::floor(v);
+      return std::min(std::max((int)std::floor(v * nbin[d]), 0), nbin[d] - 1);
+    };
+    size_t nb_tot = (size_t)nbin[0] * nbin[1] * nbin[2];
+    std::vector<int> head(nb_tot + 1, 0), items(n), bid(n);
+    for (int i = 0; i < n; i++) {
+      bid[i] = (bin_of(i, 0) * nbin[1] + bin_of(i, 1)) * nbin[2] + bin_of(i, 2);
+      head[bid[i] + 1]++;
+    }
+    for (size_t b = 0; b < nb_tot; b++) head[b + 1] += head[b];
+    {
+      std::vector<int> fill(head.begin(), head.end() - 1);
+      for (int i = 0; i < n; i++) items[fill[bid[i]]++] = i;
+    }
+    for (int i = 0; i < n; i++) {
+      int bi[3] = {bin_of(i, 0), bin_of(i, 1), bin_of(i, 2)};
+      std::set<size_t> bins;
+      for (int o0 = -1; o0 <= 1; o0++)
+        for (int o1 = -1; o1 <= 1; o1++)
+          for (int o2 = -1; o2 <= 1; o2++) {
+            int b[3] = {bi[0] + o0, bi[1] + o1, bi[2] + o2};
+            bool ok = true;
+            for (int d = 0; d < 3; d++) {
+              if (s.periodic_mask >> d & 1)
+                b[d] = ((b[d] % nbin[d]) + nbin[d]) % nbin[d];
+              else if (b[d] < 0 || b[d] >= nbin[d])
+                ok = false;
+            }
+            if (ok) bins.insert(((size_t)b[0] * nbin[1] + b[1]) * nbin[2] + b[2]);
+          }
+      for (size_t b : bins)
+        for (int k = head[b]; k < head[b + 1]; k++) {
+          int j = items[k];
+          if (j == i) continue;
+          double df[3], dv[3];
+          for (int d = 0; d < 3; d++) {
+            df[d] = f[3 * j + d] - f[3 * i + d];
+            if (s.periodic_mask >> d & 1) df[d] -= std::nearbyint(df[d]);
+          }
+          for (int d = 0; d < 3; d++) dv[d] = df[0] * s.cell[d] + df[1] * s.cell[3 + d] + df[2] * s.cell[6 + d];
+          double r2 = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
+          if (r2 < rc2) {
+            nbj.push_back(j);
+            nbd.insert(nbd.end(), dv, dv + 3);
+          }
+        }
+      rowptr[i + 1] = (int)nbj.size();
+    }
+    return Status::ok();
+  }
+  // every image within reach, every atom (small cells)
+  for (int i = 0; i < n; i++) {
+    for (int l = -reach[0]; l <= reach[0]; l++)
+      for (int m = -reach[1]; m <= reach[1]; m++)
+        for (int q = -reach[2]; q <= reach[2]; q++) {
+          double sh[3];
+          for (int d = 0; d < 3; d++) sh[d] = l * s.cell[d] + m * s.cell[3 + d] + q * s.cell[6 + d];
+          for (int j = 0; j < n; j++) {
+            if (j == i && l == 0 && m == 0 && q == 0) continue;
+            double dv[3] = {s.xyz[3 * j] + sh[0] - s.xyz[3 * i], s.xyz[3 * j + 1] + sh[1] - s.xyz[3 * i + 1],
+                            s.xyz[3 * j + 2] + sh[2] - s.xyz[3 * i + 2]};
+            double r2 = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
+            if (r2 < rc2) {
+              nbj.push_back(j);
+              nbd.insert(nbd.end(), dv, dv + 3);
+            }
+          }
+        }
+    rowptr[i + 1] = (int)nbj.size();
+  }
+  return Status::ok();
+}
 }  // namespace ltb

@@ -37,4 +37,10 @@ Status build_quartz(int L, int M, int N, int lattice, double sigma_ang, uint64_t
                     ltb_system& s);
 Status build_from_coords(ltb_system& s);  // uses s.Z, s.xyz, s.cell, s.periodic_mask
 Status sk_block(int zi, int zj, double x, double y, double z, double h[16]);
+
+// All pairs (i, j, image) with 0 < |r_j + image - r_i| < rc, images along the
+// periodic cell rows as far as rc reaches.  Per atom i: entries
+// [rowptr[i], rowptr[i+1]) hold j and the displacement r_j + image - r_i.
+Status pairs_within(const ltb_system& s, double rc, std::vector<int>& rowptr,
+                    std::vector<int>& nbj, std::vector<double>& nbd);
 }  // namespace ltb

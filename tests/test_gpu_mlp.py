@@ -1,0 +1,66 @@
+"""Learned guess: HIP descriptors, forward, backward against the numpy oracle."""
+import numpy as np
+import pytest
+
+import largetb_amd as L
+import oracle as O
+from oracle import mlp as OM
+from test_oracle_mlp import random_params
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("cells,sigma", [((2, 2, 2), 0.05), ((3, 2, 2), 0.15)])
+def test_descriptors_match_oracle(ctx, cells, sigma):
+    s = L.System.quartz(*cells, sigma_ang=sigma, seed=3)
+    xyz, Z, cell = O.quartz(*cells, sigma_ang=sigma, seed=3)
+    f = ctx.descriptors(s)
+    ref = OM.descriptors(xyz, Z, cell)
+    assert np.abs(f - ref).max() < 2e-5 * max(1.0, np.abs(ref).max())
+
+
+def dataset(ctx):
+    feats, Zs, seg, qtot, tgt = [], [], [0], [], []
+    rng = np.random.default_rng(0)
+    for cells in [(2, 2, 2), (2, 3, 2), (3, 2, 2)]:
+        s = L.System.quartz(*cells, sigma_ang=0.1, seed=sum(cells))
+        Z, _ = s.coords()
+        feats.append(ctx.descriptors(s))
+        Zs.append(Z)
+        seg.append(seg[-1] + s.nat)
+        qtot.append(s.nelectrons)
+        tgt.append(np.where(Z == 14, 0.03, 7.99) + 0.01 * rng.standard_normal(s.nat))
+    return (np.concatenate(feats), np.concatenate(Zs), np.array(seg, np.int32), np.array(qtot),
+            np.concatenate(tgt))
+
+
+def test_forward_loss_and_gradient_match_oracle(ctx):
+    feat, Z, seg, qtot, t = dataset(ctx)
+    m = L.Mlp(ctx, seed=7)
+    P = random_params(3, feat.astype(float), Z).astype(np.float32)
+    m.set_params(P)
+    m.set_data(feat, Z, seg, qtot, t)
+    q = m.forward()
+    qref = OM.forward(P.astype(float), feat.astype(float), Z, seg, qtot)
+    assert np.abs(q - qref).max() < 2e-4
+    for s in range(len(seg) - 1):
+        assert abs(q[seg[s]:seg[s + 1]].sum() - qtot[s]) < 1e-8
+    loss, g = m.loss_grad()
+    lref, gref = OM.loss_grad(P.astype(float), feat.astype(float), Z, seg, qtot, t)
+    assert loss == pytest.approx(lref, rel=1e-4)
+    scale = np.abs(gref).max()
+    assert np.abs(g - gref).max() < 2e-4 * scale
+    m.close()
+
+
+def test_adam_training_reduces_loss(ctx):
+    feat, Z, seg, qtot, t = dataset(ctx)
+    m = L.Mlp(ctx, seed=11)
+    m.set_data(feat, Z, seg, qtot, t)
+    m.fit_normalization()
+    hist = m.adam(300, lr=1e-3)
+    assert hist[-1] < 0.5 * hist[0]
+    s = L.System.quartz(2, 2, 2, sigma_ang=0.1, seed=6)
+    p = m.predict(s)
+    assert 2 * p.sum() == pytest.approx(s.nelectrons, abs=1e-8)
+    m.close()
