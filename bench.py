@@ -152,6 +152,36 @@ def time_to_solution(args):
     ctx.close()
 
 
+def sp2_measure(args):
+    """Dense SP2 on the 8x8x8 cell (18,432 orbitals): FP32 MFMA bulk + FP64
+    tail, against the KPE density at M=512; MFMA rate over the X^2 GEMMs."""
+    import numpy as np
+    import largetb_amd as L
+    cells = args.cells if args.cells != [12, 12, 12] else [8, 8, 8]
+    s = L.System.quartz(*cells, sigma_ang=0.05, seed=args.seed)
+    ctx = L.Context(0)
+    ctx.load(s)
+    ctx.set_populations(None)
+    ctx.sp2_dense(max_iter=3, fp64_tail=1)  # warm-up (allocations, code load)
+    t0 = time.perf_counter()
+    r = ctx.sp2_dense(max_iter=100, fp64_tail=2)
+    wall = time.perf_counter() - t0
+    lo, hi = ctx.gershgorin()
+    ctx.rescale((hi - lo) / 2, (hi + lo) / 2)
+    ctx.kpe_moments(512, L.FP64)
+    d = ctx.density(512)
+    n = s.norb
+    out = {"workload": f"dense SP2, {s.nat}-atom alpha-quartz {'x'.join(map(str, cells))}, {n} orbitals",
+           "iterations": r["iters"], "fp64_iterations": r["fp64_iters"], "seconds": round(wall, 3),
+           "gemm_seconds": round(r["gemm_seconds"], 4),
+           "gemm_tflops_alg": round(2.0 * n ** 3 * r["iters"] / r["gemm_seconds"] / 1e12, 2),
+           "gemm_tflops_exec": round(r["gemm_flops"] / r["gemm_seconds"] / 1e12, 2),
+           "trace_minus_nocc": r["trace"] - s.nelectrons / 2, "idem": r["idem"],
+           "max_dpop_vs_kpe_M512": float(np.abs(r["pop"] - d["pop"]).max())}
+    print(json.dumps(out))
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,6 +197,7 @@ def main():
     ap.add_argument("--tiling", type=str, default="",
                     help="explicit tiling (implies --no-autotune): chunk,V,schedule,blocks_per_cu,nt")
     ap.add_argument("--no-ablation", action="store_true")
+    ap.add_argument("--sp2", action="store_true", help="dense SP2 measurement (8x8x8 by default)")
     ap.add_argument("--tts", action="store_true",
                     help="SCC time-to-solution: plain FP64 neutral start vs learned guess + mixed")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -177,6 +208,9 @@ def main():
         return
     if args.tts:
         time_to_solution(args)
+        return
+    if args.sp2:
+        sp2_measure(args)
         return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU sharding is design-only "

@@ -176,7 +176,7 @@ int ltb_kpe_apply(ltb_ctx* ctx, int M, const double* w, int ncols, const double*
 typedef struct {
   int iterations, fp64_iterations;
   double trace;   /* tr X at the end (per spin)                  */
-  double idem;    /* |tr X - tr X^2| at the end                  */
+  double idem;    /* |tr X - tr X^2| of the final iterate          */
   double gemm_seconds; /* time inside the X^2 products          */
   double gemm_flops;   /* 2 n^3 per product, summed             */
 } ltb_sp2_result;

@@ -173,6 +173,24 @@ __global__ void k_sp2_init(const int* __restrict__ blkrow, const int* __restrict
   X[(size_t)(4 * A + r) * N + 4 * B + s] = (T)x;
 }
 
+// tr(X^2) = sum_ij X_ij^2 for symmetric X (block partials, FP64)
+template <typename T>
+__global__ __launch_bounds__(256) void k_frob2(const T* __restrict__ X, size_t n, double* __restrict__ part) {
+  double s = 0;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+    const double x = (double)X[i];
+    s += x * x;
+  }
+  __shared__ double sh[256];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
 __global__ void k_f32_to_f64(const float* __restrict__ a, double* __restrict__ b, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -226,14 +244,29 @@ int sp2_dense_impl(ltb_ctx* c, double Ne, int max_iter, int fp64_tail, double* d
   }
   double trX = (emax * norb - trH) / (emax - emin);
   dim3 grid(N / BM, N / BM), block(256);
-  auto run_phase = [&](auto tag, int max_it, bool stall_stop, int* iters, double* idem_out) -> int {
+  // branch rule: the reference's (square iff 2 tr X > Ne, computeAlphas :1277-1281)
+  // in the bulk; in the FP64 tail after an FP32 bulk, Niklasson's rule (square
+  // iff |tr X^2 - Nocc| < |2 tr X - tr X^2 - Nocc|), which also contracts
+  // eigenvalues that FP32 rounding left slightly outside [0, 1].
+  auto run_phase = [&](auto tag, int max_it, bool stall_stop, bool nearest_rule, int* iters,
+                       double* idem_out) -> int {
     using T = decltype(tag);
     T* A = (T*)c->d_sa;
     T* B = (T*)c->d_sb;
     double idem_hist[3] = {INFINITY, INFINITY, INFINITY};
     int it = 0;
     for (; it < max_it; it++) {
-      const int square = 2.0 * trX > Ne ? 1 : 0;
+      int square = 2.0 * trX > Ne ? 1 : 0;
+      if (nearest_rule) {
+        const int nb = 1024;
+        hipLaunchKernelGGL((k_frob2<T>), dim3(nb), dim3(256), 0, c->stream, (const T*)A, n2, c->d_red + 8);
+        std::vector<double> part(nb);
+        LTB_HIP(c, hipMemcpyAsync(part.data(), c->d_red + 8, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
+        LTB_HIP(c, hipStreamSynchronize(c->stream));
+        double t2 = 0;
+        for (double x : part) t2 += x;
+        square = std::fabs(t2 - nocc) < std::fabs(2.0 * trX - t2 - nocc) ? 1 : 0;
+      }
       LTB_HIP(c, hipMemsetAsync(d_tr, 0, sizeof(double), c->stream));
       LTB_HIP(c, hipEventRecord(e0, c->stream));
       hipLaunchKernelGGL((k_sp2_gemm<T>), grid, block, 0, c->stream, A, B, N, square, d_tr);
@@ -279,13 +312,13 @@ int sp2_dense_impl(ltb_ctx* c, double Ne, int max_iter, int fp64_tail, double* d
     hipLaunchKernelGGL((k_sp2_init<float>), dim3(ginit), dim3(256), 0, c->stream, c->d_blkrow,
                        c->d_col, c->d_h0, c->d_v, c->nnzb, emin, emax, N, (float*)c->d_sa);
     LTB_HIP(c, hipGetLastError());
-    if ((st2 = run_phase(float(0), max_iter, true, &it32, &idem))) return st2;
+    if ((st2 = run_phase(float(0), max_iter, true, false, &it32, &idem))) return st2;
     if (fp64_tail > 0) {
       hipLaunchKernelGGL(k_f32_to_f64, dim3(4096), dim3(256), 0, c->stream, (const float*)c->d_sa,
                          (double*)c->d_sb, n2);
       LTB_HIP(c, hipGetLastError());
       std::swap(c->d_sa, c->d_sb);
-      if ((st2 = run_phase(double(0), fp64_tail, false, &it64, &idem))) return st2;
+      if ((st2 = run_phase(double(0), fp64_tail, false, true, &it64, &idem))) return st2;
     }
   } else {
     // FP64 throughout (the reference's precision), fp64_tail is the iteration cap
@@ -293,7 +326,22 @@ int sp2_dense_impl(ltb_ctx* c, double Ne, int max_iter, int fp64_tail, double* d
     hipLaunchKernelGGL((k_sp2_init<double>), dim3(ginit), dim3(256), 0, c->stream, c->d_blkrow,
                        c->d_col, c->d_h0, c->d_v, c->nnzb, emin, emax, N, (double*)c->d_sa);
     LTB_HIP(c, hipGetLastError());
-    if ((st2 = run_phase(double(0), fp64_tail, true, &it64, &idem))) return st2;
+    if ((st2 = run_phase(double(0), fp64_tail, true, false, &it64, &idem))) return st2;
+  }
+  // idempotency of the final iterate: |tr X - tr X^2|, tr X^2 = ||X||_F^2
+  {
+    const int nb = 1024;
+    const bool f64 = fp64_tail > 0 || max_iter == 0;
+    if (f64)
+      hipLaunchKernelGGL((k_frob2<double>), dim3(nb), dim3(256), 0, c->stream, (const double*)c->d_sa, n2, c->d_red + 8);
+    else
+      hipLaunchKernelGGL((k_frob2<float>), dim3(nb), dim3(256), 0, c->stream, (const float*)c->d_sa, n2, c->d_red + 8);
+    std::vector<double> part(nb);
+    LTB_HIP(c, hipMemcpyAsync(part.data(), c->d_red + 8, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
+    LTB_HIP(c, hipStreamSynchronize(c->stream));
+    double t2 = 0;
+    for (double x : part) t2 += x;
+    idem = std::fabs(trX - t2);
   }
   std::vector<double> d(norb);
   double* dd = nullptr;
