@@ -126,6 +126,15 @@ typedef struct {
  * the moments of the last ltb_kpe_moments call.  pop: per-spin [natoms]. */
 int ltb_density(ltb_ctx* ctx, const ltb_fermi_params* fp, double* pop, ltb_density_result* res);
 
+/* diag(P) per orbital (the reference's diagP, computeDiagonalElements exact
+ * mode :1361-1379) at the mu of the last ltb_density call.  Needs per-orbital
+ * moments: ltb_set_kpe_orbital_moments(ctx, 1) before ltb_kpe_moments. */
+int ltb_set_kpe_orbital_moments(ltb_ctx* ctx, int on);
+/* the weights w[m] = c_m(mu) g_m (w[0] = c_0/2) of the last ltb_density call,
+ * as getKPE applies them (:1134-1166) -- input of ltb_kpe_apply */
+int ltb_kpe_weights(ltb_ctx* ctx, int M, double* w);
+int ltb_density_diag(ltb_ctx* ctx, double* diagP);
+
 typedef struct {
   int M;              /* Chebyshev moments                                       */
   double kT;          /* Hartree (config: 0.05 eV)                               */

@@ -1,0 +1,79 @@
+// ltb_petsc.hpp -- the reference's operator API (pghale/largeTB,
+// withSP2_deltaH.cpp) restated over the C ABI in ltb.h.
+//
+// The reference keeps its state in globals (:26-64) and calls PETSc for every
+// Mat/Vec operation.  Here one State object holds what those globals hold,
+// Vec is a host vector, Mat a host copy of a BSR/CSR matrix, and every entry
+// point keeps the reference's name, argument meaning and int return code
+// (0 = success, else an ltb_status).  The device work behind each call runs
+// through the C ABI; there is no CPU fallback.
+#ifndef LTB_PETSC_HPP
+#define LTB_PETSC_HPP
+
+#include <string>
+#include <vector>
+
+#include "ltb.h"
+
+namespace ltb_petsc {
+
+typedef int PetscErrorCode;
+typedef std::vector<double> Vec;
+
+struct Mat {                  // host copy; bs = 4 (BSR, H) or 1 (CSR, Gamma)
+  int nrows = 0, bs = 1;
+  std::vector<int> rowptr, col;
+  std::vector<double> val;
+};
+
+struct Settings {
+  int L = 1, M = 1, N = 1;              // argv L M N (:1747-1756)
+  int lattice = LTB_LATTICE_CONFIG;
+  double sigma_ang = 0.0;
+  unsigned long long seed = 1;
+  int moments = 200;                    // getKPE M (:1094)
+  double kT_ev = 0.05;                  // <= 0: the reference's step function (:1100-1103)
+  int quad_nodes = 8192;
+  int maxsp2 = 1;                       // computeDensityMatrix (:1402)
+  int precision = LTB_FP64;
+  double mix = 0.2;                     // linear charge mixing (replaces mixRatio = 0.0, :56)
+  double tol = 1e-6;                    // |dq|inf in e (reference: 0.1, :1839)
+  int max_scf = 20;                     // :1820
+  int device = 0;
+  bool verbose = true;
+};
+
+struct State {
+  Settings opt;
+  ltb_system* sys = nullptr;
+  ltb_ctx* ctx = nullptr;
+  int Natoms = 0, nSize = 0;
+  double Nelectrons = 0;
+  double muval = 0;                     // scaled chemical potential (:43)
+  double a = 1, b = 0;                  // H~ = (H - b)/a
+  double eband = 0;                     // 2 Tr(P H), Hartree
+  Mat H, Gamma_ab;
+  Vec neutralCharges, chargeOnAtom, previousCharges, netCharges, gammaq, diagP;
+  Vec mixedCharges;                     // populations H_total is built from
+  std::vector<double> alphavals;
+  std::string last_error;
+};
+
+PetscErrorCode arrangeCoordinates(State& s);                         // :669-738
+PetscErrorCode initializeForH(State& s);                             // :786-833
+PetscErrorCode constructH(State& s);                                 // :969-1066
+PetscErrorCode initializeForSCF(State& s);                           // :854-967
+PetscErrorCode GershGorinParallel(State& s, double* mn, double* mx); // :144-209
+PetscErrorCode runKPM(State& s);                                     // :1528-1723
+PetscErrorCode getKPE(State& s, const Vec& in, Vec& out);            // :1068-1185
+PetscErrorCode getSP2(State& s, const Vec& in, Vec& out, int localsp2max, int currentsp2);  // :1187-1215
+PetscErrorCode computeAlphas(State& s);                              // :1217-1294
+PetscErrorCode computeDiagonalElements(State& s);                    // :1296-1399
+PetscErrorCode computeDensityMatrix(State& s);                       // :1401-1411
+PetscErrorCode constructdeltaH(State& s);                            // :1414-1487
+PetscErrorCode updateHtotal(State& s);                               // :1489-1526
+PetscErrorCode scf(State& s, int* iterations, double* delta);        // main loop :1815-1840
+void clearAll(State& s);                                             // :741-784
+
+}  // namespace ltb_petsc
+#endif

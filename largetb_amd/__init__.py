@@ -107,6 +107,9 @@ SIGNATURES = {
     "ltb_kpe_apply": (C.c_int, [VP, C.c_int, PD, C.c_int, PD, PD, C.c_int]),
     "ltb_sp2_dense": (C.c_int, [VP, C.c_double, C.c_int, C.c_int, PD, PD, C.POINTER(Sp2Result)]),
     "ltb_kpe_stats": (C.c_int, [VP, PI, PD, PD]),
+    "ltb_set_kpe_orbital_moments": (C.c_int, [VP, C.c_int]),
+    "ltb_density_diag": (C.c_int, [VP, PD]),
+    "ltb_kpe_weights": (C.c_int, [VP, C.c_int, PD]),
     "ltb_set_kpe_timing": (C.c_int, [VP, C.c_int]),
     "ltb_set_kpe_tiling": (C.c_int, [VP, C.POINTER(KpeTiling)]),
     "ltb_get_kpe_tiling": (C.c_int, [VP, C.c_int, C.POINTER(KpeTiling)]),
@@ -312,6 +315,19 @@ class Context:
                     dq=r.dq, mu_ha=r.mu_ha, eband=r.eband, eband_fp=r.eband_fp,
                     graph_launches=r.graph_launches, seconds=r.seconds,
                     hist=hist[:r.iterations])
+
+    def keep_orbital_moments(self, on=True):
+        self._check(lib().ltb_set_kpe_orbital_moments(self._h, 1 if on else 0))
+
+    def kpe_weights(self, M):
+        w = np.zeros(M)
+        self._check(lib().ltb_kpe_weights(self._h, M, _d(w)))
+        return w
+
+    def density_diag(self):
+        d = np.zeros(self.norb)
+        self._check(lib().ltb_density_diag(self._h, _d(d)))
+        return d
 
     def kpe_apply(self, M, w, X, precision=FP64):
         X = np.ascontiguousarray(X, np.float64)

@@ -14,7 +14,8 @@ LIB = os.path.join(HERE, "libltb.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["ltb_capi.hip", "cheb.hip", "sp2.hip", "apply.hip", "mlp.hip"]
-CPP_SOURCES = ["system.cpp"]
+CPP_SOURCES = ["system.cpp", "petsc_lite.cpp"]
+DRIVER = os.path.join(HERE, "ltb_scf")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result", "-Wno-unused-variable", "-Wno-unused-value"]
@@ -54,6 +55,11 @@ def build(jobs=4):
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    # the reference-style SCC driver, linked against the library beside it
+    tmpd = DRIVER + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-o", tmpd, os.path.join(CSRC, "scf_main.cpp"),
+          "-L" + HERE, "-lltb", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + "/opt/rocm/lib"])
+    os.replace(tmpd, DRIVER)
     return LIB
 
 

@@ -240,7 +240,8 @@ __global__ __launch_bounds__(256) void k_cheb_reduce(const double* __restrict__ 
                                                      int sweep_W, int slots, int ld, int nvalid,
                                                      int j0, int n, int M,
                                                      double* __restrict__ mu1col,
-                                                     double* __restrict__ mom, int nat) {
+                                                     double* __restrict__ mom, int nat,
+                                                     double* __restrict__ momorb, int norb) {
   const int lane = threadIdx.x & 63, slice = threadIdx.x >> 6;
   const int c = blockIdx.x * 64 + lane;
   const int base = sweep_W > 0 ? ((c / sweep_W) & 7) * slots : 0;
@@ -265,6 +266,12 @@ __global__ __launch_bounds__(256) void k_cheb_reduce(const double* __restrict__ 
     modd = 2.0 * sxy - mu1col[c];
   }
   double meven = 2.0 * syy - (valid ? 1.0 : 0.0);
+  if (momorb && valid) {  // per-orbital moments [M+1][norb] (diag(P) per orbital)
+    const int j = j0 + c;
+    if (n == 0) momorb[j] = 1.0;
+    if (2 * n + 1 <= M) momorb[(size_t)(2 * n + 1) * norb + j] = modd;
+    if (2 * n + 2 <= M) momorb[(size_t)(2 * n + 2) * norb + j] = meven;
+  }
   modd += __shfl_xor(modd, 1);
   modd += __shfl_xor(modd, 2);
   meven += __shfl_xor(meven, 1);
@@ -395,6 +402,9 @@ int ensure_buffers(ltb_ctx* c, const Plan& p, int M) {
                             (size_t)(M + 1) * c->nat * sizeof(double) + (size_t)(M + 1) * sizeof(double))))
       return st;
     c->d_momtot = c->d_mom + (size_t)(M + 1) * c->nat;
+    if (c->keep_orbital &&
+        (st = ensure_buffer(c, (void**)&c->d_momorb, &c->momorbbytes, (size_t)(M + 1) * c->norb * sizeof(double))))
+      return st;
   }
   return LTB_OK;
 }
@@ -451,7 +461,8 @@ int run_moments(ltb_ctx* c, int M) {
       launches++;
       hipLaunchKernelGGL(k_cheb_reduce, dim3(ldc / 64), dim3(256), 0, c->stream, c->d_part,
                          sweep ? pc.slots : pc.n_sg, sweep ? pc.W : 0, pc.slots, ldc, nvalid, j0, n,
-                         M, c->d_mu1, c->d_mom, c->nat);
+                         M, c->d_mu1, c->d_mom, c->nat, c->keep_orbital ? c->d_momorb : nullptr,
+                         c->norb);
       LTB_HIP(c, hipGetLastError());
     }
   }

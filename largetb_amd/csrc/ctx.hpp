@@ -49,6 +49,10 @@ struct ltb_ctx {
   double* d_mom = nullptr;      // [M+1][nat]
   double* d_momtot = nullptr;   // [M+1]
   int mom_M = -1;
+  bool keep_orbital = false;    // also keep per-orbital moments [M+1][norb]
+  double* d_momorb = nullptr;
+  size_t momorbbytes = 0;
+  int momorb_M = -1;
   size_t mombytes = 0;
   // Fermi / density
   double *d_W = nullptr, *d_X = nullptr, *d_g = nullptr, *d_c = nullptr, *d_w = nullptr;

@@ -209,6 +209,16 @@ __global__ __launch_bounds__(256) void k_coeffs(const double* __restrict__ X, in
   if (threadIdx.x == 0) w[m] = m == 0 ? c / 2.0 : c * g[m];
 }
 
+// diag(P)_j = sum_m w_m [T_m]_jj (computeDiagonalElements exact mode, :1361-1379)
+__global__ void k_diagp(const double* __restrict__ momorb, const double* __restrict__ w, int M,
+                        int norb, double* __restrict__ d) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= norb) return;
+  double s = 0;
+  for (int m = 0; m < M; m++) s += w[m] * momorb[(size_t)m * norb + j];
+  d[j] = s;
+}
+
 // computeDiagonalElements (exact) + orbital2Atom (:1385): p_A = sum_m w_m d_m(A)
 __global__ void k_populations(const double* __restrict__ mom, const double* __restrict__ w, int M,
                               int nat, double* __restrict__ pop) {
@@ -408,7 +418,7 @@ void free_all(ltb_ctx* c) {
   void* ptrs[] = {c->d_rowptr, c->d_col, c->d_blkrow, c->d_h0, c->d_ht64, c->d_ht32, c->d_offabs,
                   c->d_diag0, c->d_grow, c->d_gcol, c->d_gval, c->d_p0, c->d_pin, c->d_pout,
                   c->d_v, c->d_sc, c->d_red, c->d_xa, c->d_xb, c->d_part, c->d_mu1, c->d_mom,
-                  c->d_W, c->d_X, c->d_g, c->d_c, c->d_w, c->d_sa, c->d_sb};
+                  c->d_W, c->d_X, c->d_g, c->d_c, c->d_w, c->d_sa, c->d_sb, c->d_momorb};
   for (void* p : ptrs)
     if (p) hipFree(p);
 }
@@ -677,6 +687,7 @@ int ltb_kpe_moments(ltb_ctx* c, int M, int precision, double* atom_moments, doub
   if (M < 2 || M > 1 << 16) return set_err(c, LTB_ERR_ARG, "M must be in [2, 65536]");
   if (precision != LTB_FP64 && precision != LTB_FP32_ACC64) return set_err(c, LTB_ERR_ARG, "bad precision");
   if ((st = kpe_moments_impl(c, M, precision))) return st;
+  c->momorb_M = c->keep_orbital ? M : -1;
   if (atom_moments)
     LTB_HIP(c, hipMemcpyAsync(atom_moments, c->d_mom, sizeof(double) * (size_t)(M + 1) * c->nat,
                               hipMemcpyDeviceToHost, c->stream));
@@ -802,6 +813,37 @@ int ltb_scc_run(ltb_ctx* c, const ltb_scc_params* prm, const double* guess, doub
   if (st == LTB_NOT_CONVERGED)
     set_err(c, LTB_NOT_CONVERGED, "SCC not converged after " + std::to_string(prm->max_iter) + " iterations");
   return st;
+}
+
+int ltb_set_kpe_orbital_moments(ltb_ctx* c, int on) {
+  if (!c) return LTB_ERR_ARG;
+  c->keep_orbital = on != 0;
+  return LTB_OK;
+}
+
+int ltb_density_diag(ltb_ctx* c, double* diagP) {
+  int st = need_system(c);
+  if (st) return st;
+  if (!diagP) return set_err(c, LTB_ERR_ARG, "null output");
+  if (c->momorb_M < 0 || c->momorb_M < c->fd_M || c->fd_M <= 0)
+    return set_err(c, LTB_ERR_STATE, "needs ltb_set_kpe_orbital_moments(1), ltb_kpe_moments, ltb_density");
+  double* d = nullptr;
+  LTB_HIP(c, hipMalloc(&d, sizeof(double) * c->norb));
+  hipLaunchKernelGGL(k_diagp, dim3(grid_for(c->norb, 256)), dim3(256), 0, c->stream, c->d_momorb, c->d_w,
+                     c->fd_M, c->norb, d);
+  LTB_HIP(c, hipMemcpyAsync(diagP, d, sizeof(double) * c->norb, hipMemcpyDeviceToHost, c->stream));
+  LTB_HIP(c, hipStreamSynchronize(c->stream));
+  (void)hipFree(d);
+  return LTB_OK;
+}
+
+int ltb_kpe_weights(ltb_ctx* c, int M, double* w) {
+  int st = need_system(c);
+  if (st) return st;
+  if (!w || M != c->fd_M || !c->d_w) return set_err(c, LTB_ERR_STATE, "no ltb_density call with this M");
+  LTB_HIP(c, hipMemcpyAsync(w, c->d_w, sizeof(double) * M, hipMemcpyDeviceToHost, c->stream));
+  LTB_HIP(c, hipStreamSynchronize(c->stream));
+  return LTB_OK;
 }
 
 int ltb_kpe_stats(ltb_ctx* c, int* launches, double* secs, double* bytes) {

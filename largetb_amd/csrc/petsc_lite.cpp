@@ -1,0 +1,229 @@
+// The reference's entry points (withSP2_deltaH.cpp) over the C ABI.  Each
+// function keeps the reference's name, the role of its arguments and its
+// integer return code; the Mat/Vec work of each runs on the device.
+#include "../../include/ltb_petsc.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+
+namespace ltb_petsc {
+namespace {
+
+int fail(State& s, int code) {
+  if (code != LTB_OK) s.last_error = s.ctx ? ltb_last_error(s.ctx) : ltb_system_last_error();
+  return code;
+}
+
+#define CHK(s, call)                  \
+  do {                                \
+    int e_ = (call);                  \
+    if (e_ != LTB_OK) return fail(s, e_); \
+  } while (0)
+
+// apply getSP2 at depth `depth` to the unit vectors e_j0..e_j1-1 and return
+// their diagonal elements [P_depth]_jj (the exact mode of :1266-1275, :1361-1371)
+int diag_block(State& s, int depth, int j0, int j1, std::vector<double>& out) {
+  for (int j = j0; j < j1; j++) {
+    Vec e(s.nSize, 0.0), y;
+    e[j] = 1.0;
+    CHK(s, getSP2(s, e, y, depth, depth));
+    out[j] = y[j];
+  }
+  return LTB_OK;
+}
+
+}  // namespace
+
+PetscErrorCode arrangeCoordinates(State& s) {
+  ltb_quartz_desc d{s.opt.L, s.opt.M, s.opt.N, s.opt.lattice, s.opt.sigma_ang, s.opt.seed};
+  if (s.sys) ltb_system_destroy(s.sys);
+  s.sys = nullptr;
+  CHK(s, ltb_system_quartz(&d, &s.sys));
+  int nnzb, gnnz;
+  CHK(s, ltb_system_sizes(s.sys, &s.Natoms, &nnzb, &gnnz, &s.Nelectrons));
+  s.nSize = 4 * s.Natoms;
+  s.neutralCharges.assign(s.Natoms, 0.0);
+  CHK(s, ltb_system_neutral(s.sys, s.neutralCharges.data()));
+  s.chargeOnAtom = s.neutralCharges;
+  s.mixedCharges = s.neutralCharges;
+  s.previousCharges = s.neutralCharges;
+  return LTB_OK;
+}
+
+PetscErrorCode initializeForH(State& s) {
+  if (!s.sys) return LTB_ERR_STATE;
+  if (!s.ctx) CHK(s, ltb_create(s.opt.device, &s.ctx));
+  CHK(s, ltb_load_system(s.ctx, s.sys));
+  CHK(s, ltb_set_kpe_orbital_moments(s.ctx, 1));
+  return LTB_OK;
+}
+
+PetscErrorCode constructH(State& s) {
+  // the blocks are assembled with the structure (ltb_system_quartz); this is
+  // the host copy the reference's MatView dumps would show (:1504-1509)
+  int nnzb, gnnz;
+  double ne;
+  CHK(s, ltb_system_sizes(s.sys, &s.Natoms, &nnzb, &gnnz, &ne));
+  s.H.nrows = s.Natoms;
+  s.H.bs = 4;
+  s.H.rowptr.resize(s.Natoms + 1);
+  s.H.col.resize(nnzb);
+  s.H.val.resize(16 * (size_t)nnzb);
+  CHK(s, ltb_system_hamiltonian(s.sys, s.H.rowptr.data(), s.H.col.data(), s.H.val.data()));
+  return LTB_OK;
+}
+
+PetscErrorCode initializeForSCF(State& s) {
+  int nnzb, gnnz;
+  double ne;
+  CHK(s, ltb_system_sizes(s.sys, &s.Natoms, &nnzb, &gnnz, &ne));
+  s.Gamma_ab.nrows = s.Natoms;
+  s.Gamma_ab.bs = 1;
+  s.Gamma_ab.rowptr.resize(s.Natoms + 1);
+  s.Gamma_ab.col.resize(gnnz);
+  s.Gamma_ab.val.resize(gnnz);
+  CHK(s, ltb_system_gamma(s.sys, s.Gamma_ab.rowptr.data(), s.Gamma_ab.col.data(), s.Gamma_ab.val.data()));
+  s.diagP.assign(s.nSize, 0.0);  // initializeDiagP (:1725-1740)
+  return LTB_OK;
+}
+
+PetscErrorCode GershGorinParallel(State& s, double* mn, double* mx) {
+  CHK(s, ltb_gershgorin(s.ctx, mn, mx));
+  return LTB_OK;
+}
+
+PetscErrorCode runKPM(State& s) {
+  // bounds and rescale as :1541-1560; mu by FP64 bisection on the exact
+  // moments in place of the stochastic DOS search (:1562-1712)
+  double mn, mx;
+  CHK(s, GershGorinParallel(s, &mn, &mx));
+  s.a = (mx - mn) / 2;
+  s.b = (mx + mn) / 2;
+  CHK(s, ltb_rescale(s.ctx, s.a, s.b));
+  CHK(s, ltb_kpe_moments(s.ctx, s.opt.moments, s.opt.precision, nullptr, nullptr));
+  const double kT = s.opt.kT_ev > 0 ? s.opt.kT_ev / 27.211386 : 0.0;
+  ltb_fermi_params fp{s.opt.moments, kT, s.opt.quad_nodes, s.Nelectrons};
+  ltb_density_result r;
+  CHK(s, ltb_density(s.ctx, &fp, nullptr, &r));
+  s.muval = r.mu;
+  s.eband = r.eband;
+  return LTB_OK;
+}
+
+PetscErrorCode getKPE(State& s, const Vec& in, Vec& out) {
+  if ((int)in.size() != s.nSize) return LTB_ERR_ARG;
+  std::vector<double> w(s.opt.moments);
+  CHK(s, ltb_kpe_weights(s.ctx, s.opt.moments, w.data()));
+  out.assign(s.nSize, 0.0);
+  CHK(s, ltb_kpe_apply(s.ctx, s.opt.moments, w.data(), 1, in.data(), out.data(), LTB_FP64));
+  return LTB_OK;
+}
+
+PetscErrorCode getSP2(State& s, const Vec& in, Vec& out, int localsp2max, int currentsp2) {
+  if (currentsp2 < 0) {
+    std::fprintf(stderr, "some error with calling sp2 :: negative depth\n");  // :1208-1210
+    return LTB_ERR_ARG;
+  }
+  if (currentsp2 == 0) return getKPE(s, in, out);
+  Vec y, z;
+  CHK(s, getSP2(s, in, y, localsp2max, currentsp2 - 1));
+  CHK(s, getSP2(s, y, z, localsp2max, currentsp2 - 1));
+  const double al = s.alphavals.at(currentsp2 - 1);
+  out.resize(y.size());
+  for (size_t i = 0; i < y.size(); i++) out[i] = (1.0 + al) * y[i] - al * z[i];
+  return LTB_OK;
+}
+
+PetscErrorCode computeAlphas(State& s) {
+  for (int k = 0; k < s.opt.maxsp2; k++) {
+    double tr = 0;
+    if (k == 0) {
+      // Tr P_KPE from the exact moments (the exact unit-vector trace)
+      double ne = 0;
+      std::vector<double> w(s.opt.moments);
+      CHK(s, ltb_kpe_weights(s.ctx, s.opt.moments, w.data()));
+      std::vector<double> d(s.nSize);
+      CHK(s, ltb_density_diag(s.ctx, d.data()));
+      for (double x : d) ne += x;
+      tr = ne;
+    } else {
+      std::vector<double> d(s.nSize);
+      CHK(s, diag_block(s, k, 0, s.nSize, d));
+      for (double x : d) tr += x;
+    }
+    if (s.opt.verbose) std::printf("sp2count is %d, and trace is %lf \n", k, tr);  // :1277
+    if (std::fabs(tr * 2 - s.Nelectrons) < 0.001) break;                           // :1278
+    s.alphavals[k] = tr * 2 > s.Nelectrons ? -1.0 : 1.0;                           // :1280-1281
+  }
+  return LTB_OK;
+}
+
+PetscErrorCode computeDiagonalElements(State& s) {
+  const int depth = s.opt.maxsp2 - 1;
+  s.diagP.assign(s.nSize, 0.0);
+  if (depth == 0)
+    CHK(s, ltb_density_diag(s.ctx, s.diagP.data()));
+  else
+    CHK(s, diag_block(s, depth, 0, s.nSize, s.diagP));
+  // chargeOnAtom = orbital2Atom . diagP (:1385)
+  s.chargeOnAtom.assign(s.Natoms, 0.0);
+  for (int i = 0; i < s.nSize; i++) s.chargeOnAtom[i / 4] += s.diagP[i];
+  return LTB_OK;
+}
+
+PetscErrorCode computeDensityMatrix(State& s) {
+  s.alphavals.assign(std::max(1, s.opt.maxsp2), 0.0);
+  CHK(s, computeAlphas(s));
+  CHK(s, computeDiagonalElements(s));
+  return LTB_OK;
+}
+
+PetscErrorCode constructdeltaH(State& s) {
+  // netCharges = populations - neutral; gammaq = Gamma . netCharges; the
+  // shift enters H_total's diagonal on the device (:1424-1459)
+  s.netCharges.resize(s.Natoms);
+  for (int i = 0; i < s.Natoms; i++) s.netCharges[i] = s.mixedCharges[i] - s.neutralCharges[i];
+  CHK(s, ltb_set_populations(s.ctx, s.mixedCharges.data()));
+  s.gammaq.assign(s.Natoms, 0.0);
+  CHK(s, ltb_get_shift(s.ctx, s.gammaq.data()));
+  return LTB_OK;
+}
+
+PetscErrorCode updateHtotal(State& s) {
+  // linear mixing of the populations: mixed += mix (chargeOnAtom - mixed)
+  for (int i = 0; i < s.Natoms; i++)
+    s.mixedCharges[i] += s.opt.mix * (s.chargeOnAtom[i] - s.mixedCharges[i]);
+  return LTB_OK;
+}
+
+PetscErrorCode scf(State& s, int* iterations, double* delta) {
+  int it = 0;
+  double d = INFINITY;
+  for (it = 0; it < s.opt.max_scf; it++) {
+    CHK(s, constructdeltaH(s));
+    CHK(s, runKPM(s));
+    CHK(s, computeDensityMatrix(s));
+    d = 0;
+    for (int i = 0; i < s.Natoms; i++) d = std::max(d, 2.0 * std::fabs(s.chargeOnAtom[i] - s.mixedCharges[i]));
+    if (s.opt.verbose) std::printf("scfiter : %d , the value of the scfDelta is %e \n", it, d);  // :1835
+    s.previousCharges = s.chargeOnAtom;
+    if (d < s.opt.tol) {
+      it++;
+      break;
+    }
+    CHK(s, updateHtotal(s));
+  }
+  if (iterations) *iterations = it;
+  if (delta) *delta = d;
+  return d < s.opt.tol ? LTB_OK : LTB_NOT_CONVERGED;
+}
+
+void clearAll(State& s) {
+  if (s.ctx) ltb_destroy(s.ctx);
+  if (s.sys) ltb_system_destroy(s.sys);
+  s.ctx = nullptr;
+  s.sys = nullptr;
+}
+
+}  // namespace ltb_petsc
