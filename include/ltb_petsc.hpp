@@ -35,6 +35,7 @@ struct Settings {
   double kT_ev = 0.05;                  // <= 0: the reference's step function (:1100-1103)
   int quad_nodes = 8192;
   int maxsp2 = 1;                       // computeDensityMatrix (:1402)
+  double force_alpha = 0.0;             // != 0: every SP2 level uses this alpha instead of computeAlphas' rule
   int precision = LTB_FP64;
   double mix = 0.2;                     // linear charge mixing (replaces mixRatio = 0.0, :56)
   double tol = 1e-6;                    // |dq|inf in e (reference: 0.1, :1839)

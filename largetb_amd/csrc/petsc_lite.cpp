@@ -136,6 +136,10 @@ PetscErrorCode getSP2(State& s, const Vec& in, Vec& out, int localsp2max, int cu
 }
 
 PetscErrorCode computeAlphas(State& s) {
+  if (s.opt.force_alpha != 0.0) {
+    for (double& a : s.alphavals) a = s.opt.force_alpha;
+    return LTB_OK;
+  }
   for (int k = 0; k < s.opt.maxsp2; k++) {
     double tr = 0;
     if (k == 0) {

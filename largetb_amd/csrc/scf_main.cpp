@@ -2,7 +2,7 @@
 // main :1742-1849) over the entry points of ltb_petsc.hpp.  The trace is exact
 // here, so the 4th argument of the reference (nRandoms) is not taken.
 //   ltb_scf L M N [--sigma A] [--seed S] [--moments M] [--kT eV] [--reference]
-//                 [--maxsp2 D] [--tol Q] [--max-scf K] [--fp32]
+//                 [--maxsp2 D] [--tol Q] [--max-scf K] [--fp32] [--sp2-alpha a]
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -39,6 +39,7 @@ int main(int argc, char** argv) {
     else if (a == "--tol") s.opt.tol = std::atof(val());
     else if (a == "--max-scf") s.opt.max_scf = std::atoi(val());
     else if (a == "--fp32") s.opt.precision = LTB_FP32_ACC64;
+    else if (a == "--sp2-alpha") s.opt.force_alpha = std::atof(val());
     else {
       std::fprintf(stderr, "unknown option %s\n", a.c_str());
       return 2;

@@ -40,22 +40,36 @@ def test_scf_driver_matches_oracle(tmp_path):
     assert out["total_charge"] == pytest.approx(384.0, abs=1e-8)
 
 
+def p0_eigen():
+    o = O.System.quartz(2, 2, 2, sigma_ang=0.05, seed=1)
+    d = o.density(200)
+    e, V = np.linalg.eigh(o.dense_h())
+    c = O.fd_coeffs(200, d["mu"], 0.05 / O.HARTREE_EV / d["a"], 8192)
+    f = np.polynomial.chebyshev.chebval((e - d["b"]) / d["a"], O.kpe_weights(c, O.gmM(200)))
+    return o, V, f
+
+
 @pytest.mark.gpu
 def test_sp2_depth_one_through_getSP2(tmp_path):
-    """maxsp2 = 2: diag(P1) with P1 = P0^2 or 2 P0 - P0^2 chosen by the trace
-    rule of computeAlphas, P0 the KPE density -- applied to every unit vector
-    by the recursive getSP2, as the reference's exact mode does."""
+    """maxsp2 = 2 with alpha = -1: diag(P0^2), P0 the KPE density, applied to
+    every unit vector through the recursive getSP2 (the reference's exact
+    mode), against eigenpairs."""
+    r, out, q = run(["2", "2", "2", "--sigma", "0.05", "--seed", "1", "--moments", "200",
+                     "--maxsp2", "2", "--max-scf", "1", "--sp2-alpha", "-1"], tmp_path, timeout=600)
+    assert r.returncode in (0, 5), r.stderr
+    o, V, f = p0_eigen()
+    pop = ((V ** 2) @ (f * f)).reshape(-1, 4).sum(1)
+    assert np.abs(q - 2 * pop).max() < 1e-8
+
+
+@pytest.mark.gpu
+def test_sp2_rule_stops_when_trace_is_exact(tmp_path):
+    """computeAlphas' rule (:1278): with mu bisected on exact moments,
+    |2 tr P0 - Ne| < 1e-3, so alpha stays 0 and depth 1 returns P0."""
     r, out, q = run(["2", "2", "2", "--sigma", "0.05", "--seed", "1", "--moments", "200",
                      "--maxsp2", "2", "--max-scf", "1"], tmp_path, timeout=600)
     assert r.returncode in (0, 5), r.stderr
-    o = O.System.quartz(2, 2, 2, sigma_ang=0.05, seed=1)
-    d = o.density(200)
-    H = o.dense_h()
-    e, V = np.linalg.eigh(H)
-    x = (e - d["b"]) / d["a"]
-    c = O.fd_coeffs(200, d["mu"], 0.05 / O.HARTREE_EV / d["a"], 8192)
-    f = np.polynomial.chebyshev.chebval(x, O.kpe_weights(c, O.gmM(200)))
-    alpha = -1.0 if 2 * f.sum() > o.nelectrons else 1.0
-    f1 = (1 + alpha) * f - alpha * f * f
-    pop = ((V ** 2) @ f1).reshape(-1, 4).sum(1)
+    assert "sp2count is 0, and trace is 192.0" in r.stdout
+    o, V, f = p0_eigen()
+    pop = ((V ** 2) @ f).reshape(-1, 4).sum(1)
     assert np.abs(q - 2 * pop).max() < 1e-8
