@@ -169,6 +169,31 @@ typedef struct {
 int ltb_scc_run(ltb_ctx* ctx, const ltb_scc_params* prm, const double* p_guess, double* p_out,
                 double* hist, ltb_scc_result* res);
 
+/* ---- many independent structures in one SCC ----------------------------------
+ * The reference solves one structure per job (main :1742-1849, testjob2.pbs);
+ * a batch runs the same per-structure SCC as ltb_scc_run for every structure
+ * at once: its own Gershgorin bounds and rescale, mu, Gamma shift, mixing,
+ * FP32 -> FP64 switch and convergence, while every Chebyshev recursion step
+ * of all unconverged structures is one kernel launch (plus one for those in
+ * FP64).  Converged structures drop out of the launches. */
+typedef struct ltb_batch ltb_batch;
+
+/* concatenates the structures (each keeps its own Gamma and electron count)
+ * on ctx's device; the systems may be destroyed afterwards */
+int ltb_batch_create(ltb_ctx* ctx, int nstruct, const ltb_system* const* systems, ltb_batch** out);
+void ltb_batch_destroy(ltb_batch* b);
+/* total atoms (sum over structures) and per-structure atom offsets [nstruct+1] */
+int ltb_batch_sizes(const ltb_batch* b, int* nstruct, int* natoms_total, int* atom_offsets);
+/* p_guess / p_out: per-spin populations of all atoms, concatenated in
+ * structure order [natoms_total] (p_guess NULL = neutral).  Per structure
+ * [nstruct] (each optional): iterations, eband_fp (Hartree, as ltb_scc_result),
+ * mu_ha and status (LTB_OK, LTB_NOT_CONVERGED, LTB_ERR_NONFINITE).  total:
+ * iterations of the whole batch, those with an FP64 launch, graph replays,
+ * wall time.  Returns LTB_NOT_CONVERGED if any structure hit max_iter. */
+int ltb_batch_scc_run(ltb_batch* b, const ltb_scc_params* prm, const double* p_guess, double* p_out,
+                      int* iterations, double* eband_fp, double* mu_ha, int* status,
+                      ltb_scc_result* total);
+
 /* getKPE (:1068-1185) on a block of vectors: Y = sum_m w[m] T_m(H~) X, with
  * w = (c_0/2, c_1 g_1, ...).  X, Y: [ncols][norb] column vectors (host). */
 int ltb_kpe_apply(ltb_ctx* ctx, int M, const double* w, int ncols, const double* X, double* Y,

@@ -114,6 +114,11 @@ SIGNATURES = {
     "ltb_set_kpe_tiling": (C.c_int, [VP, C.POINTER(KpeTiling)]),
     "ltb_get_kpe_tiling": (C.c_int, [VP, C.c_int, C.POINTER(KpeTiling)]),
     "ltb_kpe_autotune": (C.c_int, [VP, C.c_int, C.c_int, C.POINTER(KpeTiling), PD]),
+    "ltb_batch_create": (C.c_int, [VP, C.c_int, C.POINTER(VP), C.POINTER(VP)]),
+    "ltb_batch_destroy": (None, [VP]),
+    "ltb_batch_sizes": (C.c_int, [VP, PI, PI, PI]),
+    "ltb_batch_scc_run": (C.c_int, [VP, C.POINTER(SccParams), PD, PD, PI, PD, PD, PI,
+                                    C.POINTER(SccResult)]),
     "ltb_mlp_num_params": (C.c_int, []),
     "ltb_mlp_create": (C.c_int, [VP, C.c_ulonglong, C.POINTER(VP)]),
     "ltb_mlp_destroy": (None, [VP]),
@@ -379,6 +384,61 @@ class Context:
 
 def _f(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Batch:
+    """Independent structures of uneven size solved together on one context:
+    each keeps its own bounds, mu, mixing, precision switch and convergence,
+    while every Chebyshev step of all of them is one launch."""
+
+    def __init__(self, ctx, systems):
+        self.ctx = ctx
+        arr = (VP * len(systems))(*[s._h for s in systems])
+        h = VP()
+        ctx._check(lib().ltb_batch_create(ctx._h, len(systems), arr, C.byref(h)))
+        self._h = h
+        S, nat = C.c_int(), C.c_int()
+        off = np.zeros(len(systems) + 1, np.int32)
+        ctx._check(lib().ltb_batch_sizes(h, C.byref(S), C.byref(nat), _i(off)))
+        self.S, self.nat, self.offsets = S.value, nat.value, off
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            lib().ltb_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def scc(self, M, guess=None, kT_ev=0.05, K=8192, mix=0.2, tol=1e-6, max_iter=200,
+            precision=FP64, switch_tol=0.0, use_graph=0, check=True, verbose=0):
+        kT = kT_ev / HARTREE_EV if kT_ev > 0 else 0.0
+        prm = SccParams(M, kT, K, mix, tol, max_iter, precision, switch_tol, use_graph, verbose)
+        pout = np.zeros(self.nat)
+        its = np.zeros(self.S, np.int32)
+        ebfp = np.zeros(self.S)
+        mu = np.zeros(self.S)
+        status = np.zeros(self.S, np.int32)
+        r = SccResult()
+        g = _d(np.ascontiguousarray(guess, np.float64)) if guess is not None else None
+        st = lib().ltb_batch_scc_run(self._h, C.byref(prm), g, _d(pout), _i(its), _d(ebfp), _d(mu),
+                                     _i(status), C.byref(r))
+        if check:
+            self.ctx._check(st)
+        return dict(status=st, pop=pout, q=2 * pout, iters=its, eband_fp=ebfp, mu_ha=mu,
+                    struct_status=status, batch_iters=r.iterations, fp64_iters=r.fp64_iterations,
+                    graph_launches=r.graph_launches, seconds=r.seconds)
+
+    def split(self, x):
+        """Per-structure views of a concatenated per-atom array."""
+        return [x[self.offsets[k]:self.offsets[k + 1]] for k in range(self.S)]
 
 
 class Mlp:

@@ -1,0 +1,720 @@
+// Many independent structures of uneven size through one SCC: every
+// structure keeps its own bounds, rescale, mu, populations, mixing and
+// convergence; every recursion step of the exact-trace KPE is one launch for
+// the whole batch.  No reference counterpart (the reference runs one structure
+// per MPI job, testjob2.pbs); per structure the math is that of ltb_scc_run.
+//
+// Layout: atoms of all structures concatenated (BSR with global block
+// columns, block-diagonal across structures).  Structure s keeps its whole
+// exact trace in flight: T_n(s) is norb_s x ld_s (ld_s = norb_s rounded up to
+// 64), row-major, at byte offset 8 xoff[s] (FP32 vectors use half the room).  A task is (s, 64-column group, 16 atoms);
+// a resident grid walks the task list, skipping converged structures.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "cheb_device.hpp"
+#include "ctx.hpp"
+#include "system.hpp"
+
+namespace {
+
+constexpr int APW = 4;  // atoms per wave per task (4 waves -> 16 atoms)
+constexpr double kPi = 3.141592653589793238463;
+enum { S_EMIN, S_EMAX, S_A, S_B, S_MU, S_NELEC, S_EBAND, S_EBFP, S_DQ, S_NE, S_BAD, NSC };
+
+// active[s]: 0 done, 1 FP32 recursion, 2 FP64 recursion; the FP32 and FP64
+// instances of this kernel each take the structures of their own precision.
+template <typename T, int MODE, bool NT>
+__global__ __launch_bounds__(256) void k_batch_step(
+    const int* __restrict__ rowptr, const int* __restrict__ col, const T* __restrict__ hv,
+    const char* __restrict__ xcur, char* __restrict__ xnext, int ntasks, const int* __restrict__ task_s,
+    const int* __restrict__ task_cg, const int* __restrict__ task_sg, const int* __restrict__ aoff,
+    const int* __restrict__ lds, const size_t* __restrict__ xoffb, const int* __restrict__ active,
+    double* __restrict__ part) {
+  constexpr int want = sizeof(T) == 4 ? 1 : 2;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
+    const int s = task_s[t];
+    if (active[s] != want) continue;
+    const int cg = task_cg[t], sg = task_sg[t];
+    const int a0 = aoff[s], nat = aoff[s + 1] - a0, ld = lds[s];
+    const T* xc = reinterpret_cast<const T*>(xcur + xoffb[s]);
+    T* xn = reinterpret_cast<T*>(xnext + xoffb[s]);
+    const int c0 = cg * 64 + lane;
+    T sxy[1] = {T(0)}, syy[1] = {T(0)};
+    const int ab = (sg * 4 + wave) * APW, ae = min(ab + APW, nat);
+    for (int A = ab; A < ae; ++A)
+      ltbdev::atom_step<T, 1, MODE, NT>(rowptr, col, hv, xc, xn, a0 + A, ld, 0, c0, 4 * nat, sxy, syy, a0);
+    ltbdev::flush_partials<T, 1>(sxy, syy, wave, lane, part, (size_t)t * 2, 64, 0);
+  }
+}
+
+// per column of every active structure: fold the task partials of step n into
+// per-atom moments (as k_cheb_reduce for one structure)
+__global__ __launch_bounds__(256) void k_batch_reduce(
+    const double* __restrict__ part, int ncols, const int* __restrict__ gc_s,
+    const int* __restrict__ gc_c, const int* __restrict__ tbegin, const int* __restrict__ nsg,
+    const int* __restrict__ aoff, const int* __restrict__ active, int n, int M,
+    double* __restrict__ mu1, double* __restrict__ mom, int nat_total) {
+  const int gc = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  int s = 0, c = 0;
+  bool use = gc < ncols;
+  if (use) {
+    s = gc_s[gc];
+    c = gc_c[gc];
+    use = active[s] != 0;
+  }
+  double sxy = 0, syy = 0;
+  bool valid = false;
+  if (use) {
+    const int nat = aoff[s + 1] - aoff[s];
+    valid = c < 4 * nat;
+    const int base = tbegin[s] + (c / 64) * nsg[s];
+    for (int g = 0; g < nsg[s]; g++) {
+      sxy += part[(size_t)(base + g) * 128 + (c & 63)];
+      syy += part[(size_t)(base + g) * 128 + 64 + (c & 63)];
+    }
+  }
+  double modd = 0, meven = 0;
+  if (use) {
+    if (n == 0) {
+      modd = sxy;
+      mu1[gc] = sxy;
+    } else {
+      modd = 2.0 * sxy - mu1[gc];
+    }
+    meven = 2.0 * syy - (valid ? 1.0 : 0.0);
+  }
+  modd += __shfl_xor(modd, 1);
+  modd += __shfl_xor(modd, 2);
+  meven += __shfl_xor(meven, 1);
+  meven += __shfl_xor(meven, 2);
+  if (use && valid && (lane & 3) == 0) {
+    const int atom = aoff[s] + (c >> 2);
+    if (n == 0) mom[atom] = 4.0;
+    if (2 * n + 1 <= M) mom[(size_t)(2 * n + 1) * nat_total + atom] = modd;
+    if (2 * n + 2 <= M) mom[(size_t)(2 * n + 2) * nat_total + atom] = meven;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_batch_momtot(const double* __restrict__ mom, int nat_total,
+                                                      const int* __restrict__ aoff,
+                                                      const int* __restrict__ active, int M,
+                                                      double* __restrict__ tot) {
+  const int m = blockIdx.x, s = blockIdx.y;
+  if (!active[s]) return;
+  double acc = 0;
+  for (int a = aoff[s] + threadIdx.x; a < aoff[s + 1]; a += 256) acc += mom[(size_t)m * nat_total + a];
+  __shared__ double sh[256];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) tot[(size_t)s * (M + 1) + m] = sh[0];
+}
+
+__global__ __launch_bounds__(256) void k_batch_gersh(const double* __restrict__ diag0,
+                                                     const double* __restrict__ offabs,
+                                                     const double* __restrict__ v,
+                                                     const int* __restrict__ aoff,
+                                                     const int* __restrict__ active,
+                                                     double* __restrict__ sc) {
+  const int s = blockIdx.x;
+  if (!active[s]) return;
+  double lo = INFINITY, hi = -INFINITY;
+  int bad = 0;
+  for (int r = 4 * aoff[s] + threadIdx.x; r < 4 * aoff[s + 1]; r += 256) {
+    const double d = diag0[r] + v[r >> 2], o = offabs[r];
+    if (!isfinite(d) || !isfinite(o)) bad = 1;
+    lo = fmin(lo, d - o);
+    hi = fmax(hi, d + o);
+  }
+  __shared__ double sl[256], sh[256];
+  __shared__ int sb;
+  if (threadIdx.x == 0) sb = 0;
+  __syncthreads();
+  if (bad) atomicOr(&sb, 1);
+  sl[threadIdx.x] = lo;
+  sh[threadIdx.x] = hi;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sl[threadIdx.x] = fmin(sl[threadIdx.x], sl[threadIdx.x + w]);
+      sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double* p = sc + (size_t)s * NSC;
+    p[S_EMIN] = sl[0];
+    p[S_EMAX] = sh[0];
+    p[S_A] = (sh[0] - sl[0]) / 2;
+    p[S_B] = (sh[0] + sl[0]) / 2;
+    p[S_BAD] = (sb || !isfinite(sl[0]) || !isfinite(sh[0])) ? 1.0 : 0.0;
+  }
+}
+
+__global__ void k_batch_rescale(const double* __restrict__ h0, const int* __restrict__ blkrow,
+                                const int* __restrict__ col, const int* __restrict__ atomst,
+                                const double* __restrict__ v, const double* __restrict__ sc,
+                                const int* __restrict__ active, int nnzb, double* __restrict__ ht64,
+                                float* __restrict__ ht32) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nnzb * 16) return;
+  const int k = idx >> 4, e = idx & 15, row = blkrow[k], s = atomst[row];
+  if (!active[s]) return;
+  const double* p = sc + (size_t)s * NSC;
+  double val = h0[idx];
+  if (col[k] == row && (e % 5) == 0) val = (val + v[row]) - p[S_B];
+  val = val / p[S_A];
+  ht64[idx] = val;
+  ht32[idx] = (float)val;
+}
+
+__device__ __forceinline__ double fermi(double x, double mu, double kT) {
+  const double t = (x - mu) / kT;
+  if (t > 0) {
+    const double e = exp(-t);
+    return e / (1.0 + e);
+  }
+  return 1.0 / (1.0 + exp(t));
+}
+
+__global__ void k_batch_weights(const double* __restrict__ tot, const double* __restrict__ g, int M,
+                                int K, const int* __restrict__ active, double* __restrict__ W,
+                                double* __restrict__ X) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x, s = blockIdx.y;
+  if (k >= K) return;
+  const double th = kPi * (k + 0.5) / K;
+  if (s == 0) X[k] = cos(th);
+  if (!active[s]) return;
+  const double* t = tot + (size_t)s * (M + 1);
+  double acc = t[0];
+  for (int m = 1; m < M; m++) acc += 2.0 * g[m] * t[m] * cos(m * th);
+  W[(size_t)s * K + k] = acc / K;
+}
+
+__global__ __launch_bounds__(1024) void k_batch_bisect(const double* __restrict__ W,
+                                                       const double* __restrict__ X, int K,
+                                                       double kT, const int* __restrict__ active,
+                                                       double* __restrict__ sc) {
+  const int s = blockIdx.x;
+  if (!active[s]) return;
+  double* p = sc + (size_t)s * NSC;
+  const double kTs = kT / p[S_A], Ne = p[S_NE];
+  __shared__ double sh[1024];
+  __shared__ double s_lo, s_hi;
+  if (threadIdx.x == 0) {
+    s_lo = -1.0;
+    s_hi = 1.0;
+  }
+  __syncthreads();
+  for (int it = 0; it < 200; it++) {
+    const double lo = s_lo, hi = s_hi, mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    double part = 0;
+    for (int k = threadIdx.x; k < K; k += 1024) part += fermi(X[k], mid, kTs) * W[(size_t)s * K + k];
+    sh[threadIdx.x] = part;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      if (2.0 * sh[0] < Ne)
+        s_lo = mid;
+      else
+        s_hi = mid;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) p[S_MU] = 0.5 * (s_lo + s_hi);
+}
+
+__global__ __launch_bounds__(256) void k_batch_coeffs(const double* __restrict__ X, int K,
+                                                      const double* __restrict__ g, int M, double kT,
+                                                      const int* __restrict__ active,
+                                                      const double* __restrict__ sc,
+                                                      double* __restrict__ w) {
+  const int m = blockIdx.x, s = blockIdx.y;
+  if (!active[s]) return;
+  const double* p = sc + (size_t)s * NSC;
+  const double mu = p[S_MU], kTs = kT / p[S_A];
+  double acc = 0;
+  for (int k = threadIdx.x; k < K; k += 256) {
+    const double th = kPi * (k + 0.5) / K;
+    acc += fermi(X[k], mu, kTs) * cos(m * th);
+  }
+  __shared__ double sh[256];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int q = 128; q > 0; q >>= 1) {
+    if ((int)threadIdx.x < q) sh[threadIdx.x] += sh[threadIdx.x + q];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double c = sh[0] * (2.0 / K);
+    w[(size_t)s * M + m] = m == 0 ? c / 2.0 : c * g[m];
+  }
+}
+
+__global__ void k_batch_pops(const double* __restrict__ mom, const double* __restrict__ w, int M,
+                             int nat_total, const int* __restrict__ atomst,
+                             const int* __restrict__ active, double* __restrict__ pop) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= nat_total) return;
+  const int s = atomst[a];
+  if (!active[s]) return;
+  double acc = 0;
+  for (int m = 0; m < M; m++) acc += w[(size_t)s * M + m] * mom[(size_t)m * nat_total + a];
+  pop[a] = acc;
+}
+
+// per structure: electrons, band energy, |dq|, first-order fixed-point band
+// energy, then mixing of the structure's populations
+__global__ __launch_bounds__(256) void k_batch_finish(
+    const double* __restrict__ tot, const double* __restrict__ w, int M, const int* __restrict__ aoff,
+    const int* __restrict__ grow, const int* __restrict__ gcol, const double* __restrict__ gval,
+    double* __restrict__ pin, const double* __restrict__ pout, double mix,
+    const int* __restrict__ active, double* __restrict__ sc) {
+  const int s = blockIdx.x;
+  if (!active[s]) return;
+  double* p = sc + (size_t)s * NSC;
+  double mx = 0, corr = 0;
+  int bad = 0;
+  for (int a = aoff[s] + threadIdx.x; a < aoff[s + 1]; a += 256) {
+    const double d = 2.0 * fabs(pout[a] - pin[a]);
+    if (!isfinite(d)) bad = 1;
+    mx = fmax(mx, d);
+    double gd = 0;
+    for (int k = grow[a]; k < grow[a + 1]; k++) gd += gval[k] * (pout[gcol[k]] - pin[gcol[k]]);
+    corr += pout[a] * gd;
+  }
+  __shared__ double smx[256], scr[256];
+  __shared__ int sb;
+  if (threadIdx.x == 0) sb = 0;
+  __syncthreads();
+  if (bad) atomicOr(&sb, 1);
+  smx[threadIdx.x] = mx;
+  scr[threadIdx.x] = corr;
+  __syncthreads();
+  for (int q = 128; q > 0; q >>= 1) {
+    if ((int)threadIdx.x < q) {
+      smx[threadIdx.x] = fmax(smx[threadIdx.x], smx[threadIdx.x + q]);
+      scr[threadIdx.x] += scr[threadIdx.x + q];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double* t = tot + (size_t)s * (M + 1);
+    const double* ws = w + (size_t)s * M;
+    double ns = 0, tph = ws[0] * t[1];
+    for (int m = 0; m < M; m++) ns += ws[m] * t[m];
+    for (int m = 1; m < M; m++) tph += ws[m] * 0.5 * (t[m + 1] + t[m - 1]);
+    p[S_NELEC] = 2.0 * ns;
+    p[S_EBAND] = 2.0 * (p[S_A] * tph + p[S_B] * ns);
+    p[S_EBFP] = p[S_EBAND] + 2.0 * scr[0];
+    p[S_DQ] = smx[0];
+    if (sb || !isfinite(p[S_EBAND])) p[S_BAD] = 1.0;
+  }
+  __syncthreads();
+  for (int a = aoff[s] + threadIdx.x; a < aoff[s + 1]; a += 256) pin[a] += mix * (pout[a] - pin[a]);
+}
+
+__global__ void k_batch_shift(const int* __restrict__ grow, const int* __restrict__ gcol,
+                              const double* __restrict__ gval, const double* __restrict__ p,
+                              const double* __restrict__ p0, const int* __restrict__ atomst,
+                              const int* __restrict__ active, double* __restrict__ v, int nat) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nat || !active[atomst[i]]) return;
+  double acc = 0;
+  for (int k = grow[i]; k < grow[i + 1]; k++) acc += gval[k] * (p[gcol[k]] - p0[gcol[k]]);
+  v[i] = acc;
+}
+
+void kernel_gmM(int M, std::vector<double>& g) {
+  std::vector<double> U(M);
+  const double alpha = kPi / (M + 2), lam = std::cos(alpha);
+  double nrm = 0;
+  for (int m = 0; m < M; m++) {
+    U[m] = std::sin((m + 1) * std::acos(lam)) / std::sin(std::acos(lam));
+    nrm += U[m] * U[m];
+  }
+  nrm = std::sqrt(nrm);
+  for (double& u : U) u /= nrm;
+  g.assign(M, 0.0);
+  for (int m = 0; m < M; m++)
+    for (int i = 0; i < M - m - 1; i++) g[m] += U[i] * U[m + i];
+}
+
+}  // namespace
+
+struct ltb_batch {
+  ltb_ctx* ctx = nullptr;
+  int S = 0, nat = 0, nnzb = 0, ntasks = 0, ncols = 0;
+  std::vector<int> aoff, lds;
+  std::vector<size_t> xoff;
+  std::vector<void*> allocs;
+  int *d_rowptr, *d_col, *d_blkrow, *d_atomst, *d_grow, *d_gcol, *d_aoff, *d_lds, *d_active;
+  int *d_task_s, *d_task_cg, *d_task_sg, *d_tbegin, *d_nsg, *d_gc_s, *d_gc_c;
+  size_t* d_xoffb;
+  double *d_h0, *d_ht64, *d_diag0, *d_offabs, *d_gval, *d_p0, *d_pin, *d_pout, *d_v, *d_sc;
+  float* d_ht32;
+  size_t xelems = 0;
+  void *d_xa = nullptr, *d_xb = nullptr;
+  size_t xbytes = 0;
+  double *d_part = nullptr, *d_mu1 = nullptr, *d_mom = nullptr, *d_tot = nullptr, *d_W = nullptr,
+         *d_X = nullptr, *d_g = nullptr, *d_w = nullptr;
+  int M = -1, K = -1;
+  std::vector<double> Ne;
+};
+
+namespace {
+
+template <typename P>
+int bmalloc(ltb_batch* b, P** p, size_t bytes) {
+  hipError_t e = hipMalloc((void**)p, std::max<size_t>(bytes, 8));
+  if (e != hipSuccess) return set_err(b->ctx, LTB_ERR_NOMEM, "batch allocation failed");
+  b->allocs.push_back((void*)*p);
+  return LTB_OK;
+}
+
+template <typename P>
+int bupload(ltb_batch* b, P** p, const std::vector<P>& h) {
+  int st = bmalloc(b, p, sizeof(P) * h.size());
+  if (st) return st;
+  if (!h.empty() && hipMemcpy(*p, h.data(), sizeof(P) * h.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return set_err(b->ctx, LTB_ERR_DEVICE, "batch upload failed");
+  return LTB_OK;
+}
+
+template <typename T, int MODE>
+void launch_step(ltb_batch* b, int grid, const void* cur, void* nxt) {
+  const T* hv = sizeof(T) == 4 ? (const T*)b->d_ht32 : (const T*)b->d_ht64;
+  hipLaunchKernelGGL((k_batch_step<T, MODE, true>), dim3(grid), dim3(256), 0, b->ctx->stream, b->d_rowptr,
+                     b->d_col, hv, (const char*)cur, (char*)nxt, b->ntasks, b->d_task_s, b->d_task_cg,
+                     b->d_task_sg, b->d_aoff, b->d_lds, b->d_xoffb, b->d_active, b->d_part);
+}
+
+template <typename T>
+void launch_step(ltb_batch* b, int mode, int grid, const void* cur, void* nxt) {
+  if (mode == 0)
+    launch_step<T, 0>(b, grid, cur, nxt);
+  else if (mode == 1)
+    launch_step<T, 1>(b, grid, cur, nxt);
+  else
+    launch_step<T, 2>(b, grid, cur, nxt);
+}
+
+// the exact-trace moments of every active structure; `precs` bit 0: some
+// structure recurses in FP32, bit 1: some in FP64
+int batch_moments(ltb_batch* b, int M, int precs) {
+  ltb_ctx* c = b->ctx;
+  const int nsteps = (M + 1) / 2;
+  const int grid = std::min(b->ntasks, 8 * std::max(c->cu_count, 8));
+  for (int n = 0; n < nsteps; n++) {
+    const int mode = n == 0 ? 0 : (n == 1 ? 1 : 2);
+    void* cur = (n & 1) ? b->d_xb : b->d_xa;
+    void* nxt = (n == 0) ? b->d_xb : ((n & 1) ? b->d_xa : b->d_xb);
+    if (precs & 1) launch_step<float>(b, mode, grid, cur, nxt);
+    if (precs & 2) launch_step<double>(b, mode, grid, cur, nxt);
+    hipLaunchKernelGGL(k_batch_reduce, dim3((b->ncols + 255) / 256), dim3(256), 0, c->stream, b->d_part,
+                       b->ncols, b->d_gc_s, b->d_gc_c, b->d_tbegin, b->d_nsg, b->d_aoff, b->d_active, n,
+                       M, b->d_mu1, b->d_mom, b->nat);
+  }
+  hipLaunchKernelGGL(k_batch_momtot, dim3(M + 1, b->S), dim3(256), 0, c->stream, b->d_mom, b->nat,
+                     b->d_aoff, b->d_active, M, b->d_tot);
+  LTB_HIP(c, hipGetLastError());
+  return LTB_OK;
+}
+
+int batch_iteration(ltb_batch* b, const ltb_scc_params* prm, int precs) {
+  ltb_ctx* c = b->ctx;
+  const int M = prm->M, K = prm->K;
+  hipLaunchKernelGGL(k_batch_shift, dim3((b->nat + 255) / 256), dim3(256), 0, c->stream, b->d_grow,
+                     b->d_gcol, b->d_gval, b->d_pin, b->d_p0, b->d_atomst, b->d_active, b->d_v, b->nat);
+  hipLaunchKernelGGL(k_batch_gersh, dim3(b->S), dim3(256), 0, c->stream, b->d_diag0, b->d_offabs, b->d_v,
+                     b->d_aoff, b->d_active, b->d_sc);
+  hipLaunchKernelGGL(k_batch_rescale, dim3((b->nnzb * 16 + 255) / 256), dim3(256), 0, c->stream, b->d_h0,
+                     b->d_blkrow, b->d_col, b->d_atomst, b->d_v, b->d_sc, b->d_active, b->nnzb, b->d_ht64,
+                     b->d_ht32);
+  int st = batch_moments(b, M, precs);
+  if (st) return st;
+  hipLaunchKernelGGL(k_batch_weights, dim3((K + 255) / 256, b->S), dim3(256), 0, c->stream, b->d_tot,
+                     b->d_g, M, K, b->d_active, b->d_W, b->d_X);
+  hipLaunchKernelGGL(k_batch_bisect, dim3(b->S), dim3(1024), 0, c->stream, b->d_W, b->d_X, K, prm->kT,
+                     b->d_active, b->d_sc);
+  hipLaunchKernelGGL(k_batch_coeffs, dim3(M, b->S), dim3(256), 0, c->stream, b->d_X, K, b->d_g, M,
+                     prm->kT, b->d_active, b->d_sc, b->d_w);
+  hipLaunchKernelGGL(k_batch_pops, dim3((b->nat + 255) / 256), dim3(256), 0, c->stream, b->d_mom, b->d_w,
+                     M, b->nat, b->d_atomst, b->d_active, b->d_pout);
+  hipLaunchKernelGGL(k_batch_finish, dim3(b->S), dim3(256), 0, c->stream, b->d_tot, b->d_w, M, b->d_aoff,
+                     b->d_grow, b->d_gcol, b->d_gval, b->d_pin, b->d_pout, prm->mix, b->d_active, b->d_sc);
+  LTB_HIP(c, hipGetLastError());
+  return LTB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ltb_batch_create(ltb_ctx* c, int nstruct, const ltb_system* const* systems, ltb_batch** out) {
+  if (!c || nstruct < 1 || !systems || !out) return LTB_ERR_ARG;
+  *out = nullptr;
+  auto* b = new ltb_batch();
+  b->ctx = c;
+  b->S = nstruct;
+  std::vector<int> rowptr{0}, col, blkrow, atomst, grow{0}, gcol;
+  std::vector<double> h0, diag0, offabs, gval, p0;
+  b->aoff.push_back(0);
+  for (int s = 0; s < nstruct; s++) {
+    const ltb_system* sy = systems[s];
+    if (!sy) return delete b, set_err(c, LTB_ERR_ARG, "null system in batch");
+    const int a0 = b->aoff.back(), n = sy->natoms;
+    for (int i = 0; i < n; i++) {
+      for (int k = sy->h_rowptr[i]; k < sy->h_rowptr[i + 1]; k++) {
+        col.push_back(a0 + sy->h_col[k]);
+        blkrow.push_back(a0 + i);
+        h0.insert(h0.end(), sy->h_val.begin() + 16 * (size_t)k, sy->h_val.begin() + 16 * (size_t)k + 16);
+      }
+      rowptr.push_back((int)col.size());
+      for (int k = sy->g_rowptr[i]; k < sy->g_rowptr[i + 1]; k++) {
+        gcol.push_back(a0 + sy->g_col[k]);
+        gval.push_back(sy->g_val[k]);
+      }
+      grow.push_back((int)gcol.size());
+      atomst.push_back(s);
+      p0.push_back(sy->p0[i]);
+      for (int a = 0; a < 4; a++) {
+        double dg = 0, off = 0;
+        for (int k = sy->h_rowptr[i]; k < sy->h_rowptr[i + 1]; k++)
+          for (int q = 0; q < 4; q++) {
+            const double v = sy->h_val[16 * (size_t)k + 4 * a + q];
+            if (sy->h_col[k] == i && q == a)
+              dg = v;
+            else
+              off += std::fabs(v);
+          }
+        diag0.push_back(dg);
+        offabs.push_back(off);
+      }
+    }
+    b->aoff.push_back(a0 + n);
+    b->Ne.push_back(sy->nelectrons);
+  }
+  b->nat = b->aoff.back();
+  b->nnzb = (int)col.size();
+  // vectors, tasks, columns
+  std::vector<int> task_s, task_cg, task_sg, tbegin, nsg, gc_s, gc_c;
+  b->xoff.push_back(0);
+  for (int s = 0; s < nstruct; s++) {
+    const int nat = b->aoff[s + 1] - b->aoff[s], norb = 4 * nat, ld = (norb + 63) / 64 * 64;
+    b->lds.push_back(ld);
+    b->xoff.push_back(b->xoff.back() + (size_t)norb * ld);
+    const int ng = (nat + 4 * APW - 1) / (4 * APW), ncg = ld / 64;
+    tbegin.push_back((int)task_s.size());
+    nsg.push_back(ng);
+    for (int cg = 0; cg < ncg; cg++)
+      for (int g = 0; g < ng; g++) {
+        task_s.push_back(s);
+        task_cg.push_back(cg);
+        task_sg.push_back(g);
+      }
+    for (int cc = 0; cc < ld; cc++) {
+      gc_s.push_back(s);
+      gc_c.push_back(cc);
+    }
+  }
+  b->xelems = b->xoff.back();
+  std::vector<size_t> xoffb(b->xoff.begin(), b->xoff.end() - 1);
+  for (size_t& o : xoffb) o *= sizeof(double);  // room for FP64 vectors; FP32 uses half
+  b->ntasks = (int)task_s.size();
+  b->ncols = (int)gc_s.size();
+  std::vector<int> ones(nstruct, 1);
+  std::vector<double> sc((size_t)nstruct * NSC, 0.0);
+  for (int s = 0; s < nstruct; s++) sc[(size_t)s * NSC + S_NE] = b->Ne[s];
+  int st;
+  if ((st = bupload(b, &b->d_rowptr, rowptr)) || (st = bupload(b, &b->d_col, col)) ||
+      (st = bupload(b, &b->d_blkrow, blkrow)) || (st = bupload(b, &b->d_atomst, atomst)) ||
+      (st = bupload(b, &b->d_grow, grow)) || (st = bupload(b, &b->d_gcol, gcol)) ||
+      (st = bupload(b, &b->d_aoff, b->aoff)) || (st = bupload(b, &b->d_lds, b->lds)) ||
+      (st = bupload(b, &b->d_active, ones)) || (st = bupload(b, &b->d_task_s, task_s)) ||
+      (st = bupload(b, &b->d_task_cg, task_cg)) || (st = bupload(b, &b->d_task_sg, task_sg)) ||
+      (st = bupload(b, &b->d_tbegin, tbegin)) || (st = bupload(b, &b->d_nsg, nsg)) ||
+      (st = bupload(b, &b->d_gc_s, gc_s)) || (st = bupload(b, &b->d_gc_c, gc_c)) ||
+      (st = bupload(b, &b->d_xoffb, xoffb)) || (st = bupload(b, &b->d_h0, h0)) ||
+      (st = bupload(b, &b->d_diag0, diag0)) || (st = bupload(b, &b->d_offabs, offabs)) ||
+      (st = bupload(b, &b->d_gval, gval)) || (st = bupload(b, &b->d_p0, p0)) ||
+      (st = bupload(b, &b->d_pin, p0)) || (st = bupload(b, &b->d_pout, p0)) ||
+      (st = bupload(b, &b->d_sc, sc)) || (st = bmalloc(b, &b->d_v, sizeof(double) * b->nat)) ||
+      (st = bmalloc(b, &b->d_ht64, sizeof(double) * 16 * b->nnzb)) ||
+      (st = bmalloc(b, &b->d_ht32, sizeof(float) * 16 * b->nnzb)) ||
+      (st = bmalloc(b, &b->d_part, sizeof(double) * 128 * (size_t)b->ntasks)) ||
+      (st = bmalloc(b, &b->d_mu1, sizeof(double) * b->ncols))) {
+    ltb_batch_destroy(b);
+    return st;
+  }
+  *out = b;
+  return LTB_OK;
+}
+
+int ltb_batch_sizes(const ltb_batch* b, int* nstruct, int* natoms_total, int* atom_offsets) {
+  if (!b) return LTB_ERR_ARG;
+  if (nstruct) *nstruct = b->S;
+  if (natoms_total) *natoms_total = b->nat;
+  if (atom_offsets) std::copy(b->aoff.begin(), b->aoff.end(), atom_offsets);
+  return LTB_OK;
+}
+
+void ltb_batch_destroy(ltb_batch* b) {
+  if (!b) return;
+  (void)hipStreamSynchronize(b->ctx->stream);
+  for (void* p : b->allocs) (void)hipFree(p);
+  for (void* p : {(void*)b->d_xa, (void*)b->d_xb, (void*)b->d_mom, (void*)b->d_tot, (void*)b->d_W,
+                  (void*)b->d_X, (void*)b->d_g, (void*)b->d_w})
+    if (p) (void)hipFree(p);
+  delete b;
+}
+
+int ltb_batch_scc_run(ltb_batch* b, const ltb_scc_params* prm, const double* guess, double* p_out,
+                      int* iters, double* eband_fp, double* mu_ha, int* status, ltb_scc_result* total) {
+  if (!b || !prm) return LTB_ERR_ARG;
+  ltb_ctx* c = b->ctx;
+  if (prm->M < 2 || !(prm->kT > 0) || prm->K < 16 || prm->max_iter < 1 || !(prm->tol > 0) ||
+      !(prm->mix > 0) || prm->mix > 1)
+    return set_err(c, LTB_ERR_ARG, "bad batch SCC parameters (Fermi-Dirac kT > 0 required)");
+  const int M = prm->M, K = prm->K, S = b->S;
+  auto t0 = std::chrono::steady_clock::now();
+  // buffers sized for the precision of the bulk (FP64 needs twice the FP32 room)
+  const size_t xb = b->xelems * sizeof(double);
+  if (b->xbytes < xb) {
+    for (void** p : {&b->d_xa, &b->d_xb}) {
+      if (*p) (void)hipFree(*p);
+      *p = nullptr;
+    }
+    LTB_HIP(c, hipMalloc(&b->d_xa, xb));
+    LTB_HIP(c, hipMalloc(&b->d_xb, xb));
+    b->xbytes = xb;
+  }
+  if (b->M != M || b->K != K) {
+    for (void** p : {(void**)&b->d_mom, (void**)&b->d_tot, (void**)&b->d_W, (void**)&b->d_X, (void**)&b->d_g,
+                     (void**)&b->d_w}) {
+      if (*p) (void)hipFree(*p);
+      *p = nullptr;
+    }
+    LTB_HIP(c, hipMalloc(&b->d_mom, sizeof(double) * (size_t)(M + 1) * b->nat));
+    LTB_HIP(c, hipMalloc(&b->d_tot, sizeof(double) * (size_t)(M + 1) * S));
+    LTB_HIP(c, hipMalloc(&b->d_W, sizeof(double) * (size_t)K * S));
+    LTB_HIP(c, hipMalloc(&b->d_X, sizeof(double) * K));
+    LTB_HIP(c, hipMalloc(&b->d_g, sizeof(double) * M));
+    LTB_HIP(c, hipMalloc(&b->d_w, sizeof(double) * (size_t)M * S));
+    std::vector<double> g;
+    kernel_gmM(M, g);
+    LTB_HIP(c, hipMemcpy(b->d_g, g.data(), sizeof(double) * M, hipMemcpyHostToDevice));
+    b->M = M;
+    b->K = K;
+  }
+  if (guess)
+    LTB_HIP(c, hipMemcpy(b->d_pin, guess, sizeof(double) * b->nat, hipMemcpyHostToDevice));
+  else
+    LTB_HIP(c, hipMemcpy(b->d_pin, b->d_p0, sizeof(double) * b->nat, hipMemcpyDeviceToDevice));
+  // per structure: 1 recursing in FP32, 2 in FP64, 0 done (converged or failed)
+  const int start = prm->precision == LTB_FP64 ? 2 : 1;
+  std::vector<int> active(S, start), it_s(S, 0), st_s(S, LTB_NOT_CONVERGED);
+  std::vector<double> sc((size_t)S * NSC);
+  int it = 0, n64 = 0, ngraph = 0;
+  hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool warmed[4] = {false, false, false, false};
+  int result = LTB_OK;
+  for (it = 0; it < prm->max_iter; it++) {
+    int precs = 0;
+    for (int s = 0; s < S; s++) precs |= active[s] == 1 ? 1 : (active[s] == 2 ? 2 : 0);
+    if (!precs) break;
+    LTB_HIP(c, hipMemcpy(b->d_active, active.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+    if (prm->use_graph && warmed[precs]) {
+      if (!exec[precs]) {
+        hipGraph_t g = nullptr;
+        LTB_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int s2 = batch_iteration(b, prm, precs);
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (s2) return s2;
+        if (e != hipSuccess) return set_err(c, LTB_ERR_DEVICE, std::string("batch graph capture: ") + hipGetErrorString(e));
+        const hipError_t e2 = hipGraphInstantiate(&exec[precs], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e2 != hipSuccess) return set_err(c, LTB_ERR_DEVICE, "batch graph instantiate failed");
+      }
+      LTB_HIP(c, hipGraphLaunch(exec[precs], c->stream));
+      ngraph++;
+    } else {
+      const int s2 = batch_iteration(b, prm, precs);
+      if (s2) return s2;
+      warmed[precs] = true;
+    }
+    if (precs & 2) n64++;
+    LTB_HIP(c, hipMemcpyAsync(sc.data(), b->d_sc, sizeof(double) * sc.size(), hipMemcpyDeviceToHost, c->stream));
+    LTB_HIP(c, hipStreamSynchronize(c->stream));
+    int n32 = 0, n64s = 0;
+    for (int s = 0; s < S; s++) {
+      if (!active[s]) continue;
+      const double* p = sc.data() + (size_t)s * NSC;
+      it_s[s]++;
+      if (p[S_BAD] != 0.0) {
+        st_s[s] = LTB_ERR_NONFINITE;
+        active[s] = 0;
+        result = LTB_ERR_NONFINITE;
+        continue;
+      }
+      // the switch rule of ltb_scc_run, per structure
+      const bool final_prec = active[s] == 2 || prm->switch_tol <= 0;
+      if (p[S_DQ] < prm->tol && final_prec) {
+        st_s[s] = LTB_OK;
+        active[s] = 0;
+        continue;
+      }
+      if (active[s] == 1 && prm->switch_tol > 0 && p[S_DQ] < prm->switch_tol) active[s] = 2;
+      (active[s] == 1 ? n32 : n64s)++;
+    }
+    if (prm->verbose) {
+      const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      fprintf(stderr, "batch scc iter %d: %d fp32 + %d fp64 of %d structures still iterating, t=%.2fs\n", it,
+              n32, n64s, S, el);
+    }
+  }
+  for (hipGraphExec_t e : exec)
+    if (e) (void)hipGraphExecDestroy(e);
+  if (p_out) LTB_HIP(c, hipMemcpy(p_out, b->d_pout, sizeof(double) * b->nat, hipMemcpyDeviceToHost));
+  int nconv = 0;
+  for (int s = 0; s < S; s++) {
+    const double* p = sc.data() + (size_t)s * NSC;
+    if (iters) iters[s] = it_s[s];
+    if (eband_fp) eband_fp[s] = p[S_EBFP];
+    if (mu_ha) mu_ha[s] = p[S_A] * p[S_MU] + p[S_B];
+    if (status) status[s] = st_s[s];
+    nconv += st_s[s] == LTB_OK;
+  }
+  if (total) {
+    total->iterations = it;
+    total->fp64_iterations = n64;
+    total->dq = 0;
+    total->mu_ha = 0;
+    total->eband = 0;
+    total->eband_fp = 0;
+    total->graph_launches = ngraph;
+    total->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  if (result) return set_err(c, result, "non-finite value in a batched structure");
+  if (nconv < S) return set_err(c, LTB_NOT_CONVERGED, std::to_string(S - nconv) + " structures not converged");
+  return LTB_OK;
+}
+
+}  // extern "C"
