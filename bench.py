@@ -182,6 +182,71 @@ def sp2_measure(args):
     ctx.close()
 
 
+def batch_measure(args):
+    """Many independent uneven cells in one batched SCC (2x2x2 .. 5x5x4 quartz,
+    sigma 0.02-0.15 A from the seed, M=256), FP32 bulk with FP64 finish, every
+    cell to |dq| < 1e-6 e; against the same cells solved one at a time."""
+    import numpy as np
+    import largetb_amd as L
+    from largetb_amd.learned import batch_structures
+    cells = batch_structures(args.batch, seed=args.seed)
+    t0 = time.perf_counter()
+    systems = [L.System.quartz(l, m, n, sigma_ang=sg, seed=sd) for l, m, n, sg, sd in cells]
+    nat = sum(s.nat for s in systems)
+    log(f"{len(systems)} cells, {nat} atoms, built in {time.perf_counter() - t0:.1f} s")
+    ctx = L.Context(0)
+    b = L.Batch(ctx, systems)
+    M = args.M if args.M != 512 else 256
+    t0 = time.perf_counter()
+    tiling, tune_gbs = b.autotune(L.FP32_ACC64, 3)
+    log(f"batch autotune {time.perf_counter() - t0:.2f} s: {tiling} at {tune_gbs:.0f} GB/s")
+    kw = dict(precision=L.FP32_ACC64, switch_tol=1e-6, tol=1e-6, max_iter=400)
+    b.scc(M, max_iter=1, tol=1e-30, check=False)                       # warm-up, FP32 path
+    b.scc(M, precision=L.FP64, max_iter=1, tol=1e-30, check=False)     # and FP64 path
+    ctx.set_timing(True)
+    eager = b.scc(M, use_graph=0, **kw)
+    st = b.stats()
+    ctx.set_timing(False)
+    log(f"eager: {eager['batch_iters']} batch iterations, {eager['seconds']:.2f} s")
+    graph = b.scc(M, use_graph=1, verbose=1, **kw)
+    log(f"graph: {graph['batch_iters']} batch iterations, {graph['seconds']:.2f} s")
+    qb = b.split(graph["q"])
+    b.close()
+    # the same cells one at a time (single-structure SCC, same settings, graphs)
+    nseq = len(systems) if args.batch_seq <= 0 else min(args.batch_seq, len(systems))
+    pick = np.unique(np.linspace(0, len(systems) - 1, nseq).round().astype(int))
+    tseq, err, eerr = 0.0, 0.0, 0.0
+    for k in pick:
+        ctx.load(systems[k])
+        t1 = time.perf_counter()
+        r = ctx.scc(M, use_graph=1, **kw)
+        tseq += time.perf_counter() - t1
+        ref = ctx.scc(M, precision=L.FP64, tol=1e-6, max_iter=400, use_graph=1)
+        err = max(err, float(np.abs(qb[k] - ref["q"]).max()))
+        eerr = max(eerr, abs(graph["eband_fp"][k] - ref["eband_fp"]) * 27.211386 / systems[k].nat)
+    ctx.close()
+    seq_all = tseq / len(pick) * len(systems)
+    achieved = st["bytes"] / st["seconds"] / 1e9 if st["seconds"] > 0 else None
+    out = {"workload": f"batched SCC, {len(systems)} uneven alpha-quartz cells (2x2x2..5x5x4), "
+                       f"{nat} atoms, exact-trace KPE M={M}, FP32 bulk + FP64 finish to 1e-6 e",
+           "cells": len(systems), "atoms": nat, "M": M, "seed": args.seed,
+           "batch_iterations": graph["batch_iters"], "fp64_batch_iterations": graph["fp64_iters"],
+           "cell_iterations_min_max": [int(graph["iters"].min()), int(graph["iters"].max())],
+           "seconds_graph": round(graph["seconds"], 3), "seconds_eager": round(eager["seconds"], 3),
+           "cells_per_s": round(len(systems) / graph["seconds"], 2),
+           "one_at_a_time_seconds": round(seq_all, 3),
+           "one_at_a_time_sample": f"{len(pick)} of {len(systems)} cells, scaled to all",
+           "speedup_vs_one_at_a_time": round(seq_all / graph["seconds"], 2),
+           "max_dq_vs_fp64_e": err, "max_deband_vs_fp64_ev_per_atom": eerr,
+           "roofline": {"bound": "hbm", "kernel": "k_batch_step", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+                        "achieved": round(achieved, 1) if achieved else None,
+                        "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                        "launches": st["launches"],
+                        "mean_launch_us": round(st["seconds"] / max(st["launches"], 1) * 1e6, 2),
+                        "tiling": {k: tiling[k] for k in ("blocks_per_cu", "atoms_per_wave", "nontemporal")}}}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,6 +265,10 @@ def main():
     ap.add_argument("--sp2", action="store_true", help="dense SP2 measurement (8x8x8 by default)")
     ap.add_argument("--tts", action="store_true",
                     help="SCC time-to-solution: plain FP64 neutral start vs learned guess + mixed")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="batched SCC over this many uneven cells (M=256 unless --M is given)")
+    ap.add_argument("--batch-seq", type=int, default=32,
+                    help="cells re-solved one at a time for the comparison (0: all)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 1 GiB device-to-device copies (counter calibration)")
     args = ap.parse_args()
@@ -211,6 +280,9 @@ def main():
         return
     if args.sp2:
         sp2_measure(args)
+        return
+    if args.batch:
+        batch_measure(args)
         return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU sharding is design-only "

@@ -194,6 +194,14 @@ int ltb_batch_scc_run(ltb_batch* b, const ltb_scc_params* prm, const double* p_g
                       int* iterations, double* eband_fp, double* mu_ha, int* status,
                       ltb_scc_result* total);
 
+/* step-kernel launches, their summed duration (s, HIP events on the context
+ * stream) and algorithmic bytes in the eager iterations of the last
+ * ltb_batch_scc_run while ltb_set_kpe_timing is on (graph replays are not
+ * timed).  Bytes per structure and step: the valid columns of the vectors
+ * read by the gathers, read as t_{n-1} and written as t_{n+1} (1, 2, 3
+ * streams for steps 0, 1, >1) plus its H~ blocks, columns and row pointers. */
+int ltb_batch_stats(const ltb_batch* b, long long* launches, double* seconds, double* bytes);
+
 /* getKPE (:1068-1185) on a block of vectors: Y = sum_m w[m] T_m(H~) X, with
  * w = (c_0/2, c_1 g_1, ...).  X, Y: [ncols][norb] column vectors (host). */
 int ltb_kpe_apply(ltb_ctx* ctx, int M, const double* w, int ncols, const double* X, double* Y,
@@ -249,6 +257,19 @@ int ltb_get_kpe_tiling(ltb_ctx* ctx, int precision, ltb_kpe_tiling* out);
  * algorithmic GB/s. */
 int ltb_kpe_autotune(ltb_ctx* ctx, int precision, int trial_steps, ltb_kpe_tiling* best,
                      double* best_gbs);
+
+/* Tiling of a batch's step kernel.  Column groups are fixed (256 columns, 4
+ * per lane in FP32 and 2 x 2 in FP64, then 64-column groups for the rest);
+ * atoms_per_wave, blocks_per_cu and nontemporal are taken (0 / -1 keep the
+ * current value).  No reference counterpart: the reference has one structure
+ * per job and no tiling. */
+int ltb_batch_set_tiling(ltb_batch* b, const ltb_kpe_tiling* t);
+int ltb_batch_get_tiling(const ltb_batch* b, ltb_kpe_tiling* out);
+/* Time `trial_steps` steps of every candidate (atoms_per_wave 4/8/16 x
+ * blocks_per_cu 2/4/8 x nontemporal 0/1) over all structures at their neutral
+ * start, keep the fastest and report its algorithmic GB/s. */
+int ltb_batch_autotune(ltb_batch* b, int precision, int trial_steps, ltb_kpe_tiling* best,
+                       double* best_gbs);
 
 /* ---- learned SCC starting guess ----------------------------------------------
  * No reference counterpart: the reference starts every SCC from the neutral

@@ -117,6 +117,10 @@ SIGNATURES = {
     "ltb_batch_create": (C.c_int, [VP, C.c_int, C.POINTER(VP), C.POINTER(VP)]),
     "ltb_batch_destroy": (None, [VP]),
     "ltb_batch_sizes": (C.c_int, [VP, PI, PI, PI]),
+    "ltb_batch_stats": (C.c_int, [VP, C.POINTER(C.c_longlong), PD, PD]),
+    "ltb_batch_set_tiling": (C.c_int, [VP, C.POINTER(KpeTiling)]),
+    "ltb_batch_get_tiling": (C.c_int, [VP, C.POINTER(KpeTiling)]),
+    "ltb_batch_autotune": (C.c_int, [VP, C.c_int, C.c_int, C.POINTER(KpeTiling), PD]),
     "ltb_batch_scc_run": (C.c_int, [VP, C.POINTER(SccParams), PD, PD, PI, PD, PD, PI,
                                     C.POINTER(SccResult)]),
     "ltb_mlp_num_params": (C.c_int, []),
@@ -435,6 +439,25 @@ class Batch:
         return dict(status=st, pop=pout, q=2 * pout, iters=its, eband_fp=ebfp, mu_ha=mu,
                     struct_status=status, batch_iters=r.iterations, fp64_iters=r.fp64_iterations,
                     graph_launches=r.graph_launches, seconds=r.seconds)
+
+    def tiling(self):
+        t = KpeTiling()
+        self.ctx._check(lib().ltb_batch_get_tiling(self._h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in KpeTiling._fields_}
+
+    def set_tiling(self, atoms_per_wave=0, blocks_per_cu=0, nontemporal=-1):
+        t = KpeTiling(0, 0, 0, blocks_per_cu, atoms_per_wave, nontemporal)
+        self.ctx._check(lib().ltb_batch_set_tiling(self._h, C.byref(t)))
+
+    def autotune(self, precision=FP32_ACC64, trial_steps=3):
+        t, g = KpeTiling(), C.c_double()
+        self.ctx._check(lib().ltb_batch_autotune(self._h, precision, trial_steps, C.byref(t), C.byref(g)))
+        return {k: getattr(t, k) for k, _ in KpeTiling._fields_}, g.value
+
+    def stats(self):
+        n, sec, by = C.c_longlong(), C.c_double(), C.c_double()
+        self.ctx._check(lib().ltb_batch_stats(self._h, C.byref(n), C.byref(sec), C.byref(by)))
+        return dict(launches=n.value, seconds=sec.value, bytes=by.value)
 
     def split(self, x):
         """Per-structure views of a concatenated per-atom array."""

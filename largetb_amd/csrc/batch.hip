@@ -24,35 +24,64 @@
 
 namespace {
 
-constexpr int APW = 4;  // atoms per wave per task (4 waves -> 16 atoms)
 constexpr double kPi = 3.141592653589793238463;
 enum { S_EMIN, S_EMAX, S_A, S_B, S_MU, S_NELEC, S_EBAND, S_EBFP, S_DQ, S_NE, S_BAD, NSC };
 
+// One task = one column group of one structure x 4*apw consecutive atoms.
+// Wide groups are 256 columns (FP32: 4 columns per lane, 16-byte accesses;
+// FP64: two passes of 2 columns per lane), the structure's last columns form
+// narrow 64-column groups.  Partials: 2 x 256 doubles per task.
 // active[s]: 0 done, 1 FP32 recursion, 2 FP64 recursion; the FP32 and FP64
 // instances of this kernel each take the structures of their own precision.
+// slots[]: task per grid-stride slot (-1 = padding), laid out so that slot i
+// runs on XCD i % 8 and every task of one column group shares an XCD.
+template <typename T, int V, int MODE, bool NT>
+__device__ __forceinline__ void task_pass(const int* __restrict__ rowptr, const int* __restrict__ col,
+                                          const T* __restrict__ hv, const T* xc, T* xn, int a0, int ab,
+                                          int ae, int ld, int c0, int nvalid, int wave, int lane,
+                                          double* __restrict__ part, size_t row2, int pcol) {
+  T sxy[V], syy[V];
+#pragma unroll
+  for (int v = 0; v < V; v++) sxy[v] = syy[v] = T(0);
+  for (int A = ab; A < ae; ++A)
+    ltbdev::atom_step<T, V, MODE, NT>(rowptr, col, hv, xc, xn, a0 + A, ld, 0, c0, nvalid, sxy, syy, a0);
+  ltbdev::flush_partials<T, V>(sxy, syy, wave, lane, part, row2, 256, pcol);
+}
+
 template <typename T, int MODE, bool NT>
 __global__ __launch_bounds__(256) void k_batch_step(
     const int* __restrict__ rowptr, const int* __restrict__ col, const T* __restrict__ hv,
-    const char* __restrict__ xcur, char* __restrict__ xnext, int ntasks, const int* __restrict__ task_s,
-    const int* __restrict__ task_cg, const int* __restrict__ task_sg, const int* __restrict__ aoff,
-    const int* __restrict__ lds, const size_t* __restrict__ xoffb, const int* __restrict__ active,
-    double* __restrict__ part) {
+    const char* __restrict__ xcur, char* __restrict__ xnext, int nslots, const int* __restrict__ slots,
+    const int* __restrict__ task_s, const int* __restrict__ task_c0, const int* __restrict__ task_sg,
+    const int* __restrict__ aoff, const int* __restrict__ lds, const size_t* __restrict__ xoffb,
+    const int* __restrict__ active, double* __restrict__ part, int apw) {
   constexpr int want = sizeof(T) == 4 ? 1 : 2;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
+  for (int i = blockIdx.x; i < nslots; i += gridDim.x) {
+    const int t = slots[i];
+    if (t < 0) continue;
     const int s = task_s[t];
     if (active[s] != want) continue;
-    const int cg = task_cg[t], sg = task_sg[t];
-    const int a0 = aoff[s], nat = aoff[s + 1] - a0, ld = lds[s];
+    const int cstart = task_c0[t], sg = task_sg[t];
+    const int a0 = aoff[s], nat = aoff[s + 1] - a0, ld = lds[s], nvalid = 4 * nat;
     const T* xc = reinterpret_cast<const T*>(xcur + xoffb[s]);
     T* xn = reinterpret_cast<T*>(xnext + xoffb[s]);
-    const int c0 = cg * 64 + lane;
-    T sxy[1] = {T(0)}, syy[1] = {T(0)};
-    const int ab = (sg * 4 + wave) * APW, ae = min(ab + APW, nat);
-    for (int A = ab; A < ae; ++A)
-      ltbdev::atom_step<T, 1, MODE, NT>(rowptr, col, hv, xc, xn, a0 + A, ld, 0, c0, 4 * nat, sxy, syy, a0);
-    ltbdev::flush_partials<T, 1>(sxy, syy, wave, lane, part, (size_t)t * 2, 64, 0);
+    const int ab = min((sg * 4 + wave) * apw, nat), ae = min(ab + apw, nat);
+    const size_t row2 = (size_t)t * 2;
+    if (cstart + 256 <= (nvalid & ~255)) {
+      if constexpr (sizeof(T) == 4) {
+        task_pass<T, 4, MODE, NT>(rowptr, col, hv, xc, xn, a0, ab, ae, ld, cstart + 4 * lane, nvalid, wave,
+                                  lane, part, row2, 0);
+      } else {
+        for (int h = 0; h < 2; h++)
+          task_pass<T, 2, MODE, NT>(rowptr, col, hv, xc, xn, a0, ab, ae, ld, cstart + 128 * h + 2 * lane,
+                                    nvalid, wave, lane, part, row2, 128 * h);
+      }
+    } else {
+      task_pass<T, 1, MODE, NT>(rowptr, col, hv, xc, xn, a0, ab, ae, ld, cstart + lane, nvalid, wave, lane,
+                                part, row2, 0);
+    }
   }
 }
 
@@ -77,10 +106,14 @@ __global__ __launch_bounds__(256) void k_batch_reduce(
   if (use) {
     const int nat = aoff[s + 1] - aoff[s];
     valid = c < 4 * nat;
-    const int base = tbegin[s] + (c / 64) * nsg[s];
+    // wide 256-column groups first, then narrow 64-column groups
+    const int cw = (4 * nat) & ~255;
+    const int gidx = c < cw ? c >> 8 : (cw >> 8) + ((c - cw) >> 6);
+    const int off = c < cw ? c & 255 : (c - cw) & 63;
+    const int base = tbegin[s] + gidx * nsg[s];
     for (int g = 0; g < nsg[s]; g++) {
-      sxy += part[(size_t)(base + g) * 128 + (c & 63)];
-      syy += part[(size_t)(base + g) * 128 + 64 + (c & 63)];
+      sxy += part[(size_t)(base + g) * 512 + off];
+      syy += part[(size_t)(base + g) * 512 + 256 + off];
     }
   }
   double modd = 0, meven = 0;
@@ -366,7 +399,11 @@ struct ltb_batch {
   std::vector<size_t> xoff;
   std::vector<void*> allocs;
   int *d_rowptr, *d_col, *d_blkrow, *d_atomst, *d_grow, *d_gcol, *d_aoff, *d_lds, *d_active;
-  int *d_task_s, *d_task_cg, *d_task_sg, *d_tbegin, *d_nsg, *d_gc_s, *d_gc_c;
+  int nslots = 0;
+  ltb_kpe_tiling tiling{256, 4, LTB_SCHED_SWEEP, 8, 8, 1};  // column group, V, -, blocks/CU, atoms/wave, NT
+  int *d_slots = nullptr, *d_task_s = nullptr, *d_task_c0 = nullptr, *d_task_sg = nullptr,
+      *d_tbegin = nullptr, *d_nsg = nullptr;
+  int *d_gc_s, *d_gc_c;
   size_t* d_xoffb;
   double *d_h0, *d_ht64, *d_diag0, *d_offabs, *d_gval, *d_p0, *d_pin, *d_pout, *d_v, *d_sc;
   float* d_ht32;
@@ -377,6 +414,12 @@ struct ltb_batch {
          *d_X = nullptr, *d_g = nullptr, *d_w = nullptr;
   int M = -1, K = -1;
   std::vector<double> Ne;
+  std::vector<int> nnzb_s;
+  // step-kernel statistics of the last ltb_batch_scc_run (events only in eager
+  // iterations while the context's KPE timing is on)
+  std::vector<hipEvent_t> events;
+  long long stat_launches = 0;
+  double stat_seconds = 0, stat_bytes = 0;
 };
 
 namespace {
@@ -398,12 +441,120 @@ int bupload(ltb_batch* b, P** p, const std::vector<P>& h) {
   return LTB_OK;
 }
 
+// algorithmic bytes of the step kernels of one iteration: per structure and
+// step, the vectors read by the gathers (t_n), read (t_{n-1}) and written
+// (t_{n+1}) once over the valid columns (1, 2, 3 streams for steps 0, 1, >1),
+// plus H~ blocks, block-column indices and row pointers once
+double iteration_bytes(const ltb_batch* b, const std::vector<int>& active, int M) {
+  const int nsteps = (M + 1) / 2;
+  double bytes = 0;
+  for (int s = 0; s < b->S; s++) {
+    if (!active[s]) continue;
+    const double el = active[s] == 1 ? 4.0 : 8.0;
+    const double norb = 4.0 * (b->aoff[s + 1] - b->aoff[s]);
+    const double hb = b->nnzb_s[s] * (16.0 * el + 4.0) + 4.0 * (norb / 4 + 1);
+    for (int n = 0; n < nsteps; n++) {
+      const int streams = n == 0 ? 1 : (n == 1 ? 2 : 3);
+      bytes += streams * el * norb * norb + hb;
+    }
+  }
+  return bytes;
+}
+
+void free_tasks(ltb_batch* b) {
+  for (int** p : {&b->d_slots, &b->d_task_s, &b->d_task_c0, &b->d_task_sg, &b->d_tbegin, &b->d_nsg}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  if (b->d_part) (void)hipFree(b->d_part);
+  b->d_part = nullptr;
+}
+
+int upload_tmp(ltb_batch* b, int** p, const std::vector<int>& h) {
+  if (hipMalloc((void**)p, std::max<size_t>(sizeof(int) * h.size(), 8)) != hipSuccess)
+    return set_err(b->ctx, LTB_ERR_NOMEM, "batch task allocation failed");
+  if (!h.empty() && hipMemcpy(*p, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return set_err(b->ctx, LTB_ERR_DEVICE, "batch task upload failed");
+  return LTB_OK;
+}
+
+// Tasks for the batch's tiling: per structure its column groups (256-wide,
+// then 64-wide for the columns left), each split into blocks of
+// 4*atoms_per_wave atoms.  Column groups go to the least loaded XCD (load =
+// columns x atoms) so that each XCD's L2 serves the gathers of its own
+// groups; slot 8 i + x holds XCD x's i-th task.
+int build_tasks(ltb_batch* b) {
+  const int apw = b->tiling.atoms_per_wave;
+  std::vector<int> task_s, task_c0, task_sg, tbegin, nsg;
+  std::vector<std::vector<int>> xcd_tasks(8);
+  std::vector<double> xcd_load(8, 0.0);
+  for (int s = 0; s < b->S; s++) {
+    const int nat = b->aoff[s + 1] - b->aoff[s], norb = 4 * nat, ld = b->lds[s];
+    const int ng = (nat + 4 * apw - 1) / (4 * apw), cw = norb & ~255;
+    tbegin.push_back((int)task_s.size());
+    nsg.push_back(ng);
+    for (int c0 = 0; c0 < ld;) {
+      const int width = c0 < cw ? 256 : 64;
+      const int x = (int)(std::min_element(xcd_load.begin(), xcd_load.end()) - xcd_load.begin());
+      xcd_load[x] += (double)width * nat;
+      for (int g = 0; g < ng; g++) {
+        xcd_tasks[x].push_back((int)task_s.size());
+        task_s.push_back(s);
+        task_c0.push_back(c0);
+        task_sg.push_back(g);
+      }
+      c0 += width;
+    }
+  }
+  size_t per_xcd = 0;
+  for (auto& v : xcd_tasks) per_xcd = std::max(per_xcd, v.size());
+  std::vector<int> slots(8 * per_xcd, -1);
+  for (int x = 0; x < 8; x++)
+    for (size_t i = 0; i < xcd_tasks[x].size(); i++) slots[8 * i + x] = xcd_tasks[x][i];
+  free_tasks(b);
+  b->nslots = (int)slots.size();
+  b->ntasks = (int)task_s.size();
+  int st;
+  if ((st = upload_tmp(b, &b->d_slots, slots)) || (st = upload_tmp(b, &b->d_task_s, task_s)) ||
+      (st = upload_tmp(b, &b->d_task_c0, task_c0)) || (st = upload_tmp(b, &b->d_task_sg, task_sg)) ||
+      (st = upload_tmp(b, &b->d_tbegin, tbegin)) || (st = upload_tmp(b, &b->d_nsg, nsg)))
+    return st;
+  if (hipMalloc((void**)&b->d_part, sizeof(double) * 512 * (size_t)b->ntasks) != hipSuccess)
+    return set_err(b->ctx, LTB_ERR_NOMEM, "batch partials allocation failed");
+  return LTB_OK;
+}
+
+// ping-pong vector buffers with FP64 room for every structure
+int ensure_vectors(ltb_batch* b) {
+  ltb_ctx* c = b->ctx;
+  const size_t xb = b->xelems * sizeof(double);
+  if (b->xbytes >= xb) return LTB_OK;
+  for (void** p : {&b->d_xa, &b->d_xb}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  LTB_HIP(c, hipMalloc(&b->d_xa, xb));
+  LTB_HIP(c, hipMalloc(&b->d_xb, xb));
+  b->xbytes = xb;
+  return LTB_OK;
+}
+
+// resident grid: a multiple of 8 blocks keeps slot i on XCD i % 8 through the
+// grid-stride loop
+int step_grid(const ltb_batch* b) {
+  const int per_cu = std::max(1, b->tiling.blocks_per_cu);
+  const int grid = std::min(b->nslots, per_cu * std::max(b->ctx->cu_count, 8));
+  return std::max(8, grid / 8 * 8);
+}
+
 template <typename T, int MODE>
 void launch_step(ltb_batch* b, int grid, const void* cur, void* nxt) {
   const T* hv = sizeof(T) == 4 ? (const T*)b->d_ht32 : (const T*)b->d_ht64;
-  hipLaunchKernelGGL((k_batch_step<T, MODE, true>), dim3(grid), dim3(256), 0, b->ctx->stream, b->d_rowptr,
-                     b->d_col, hv, (const char*)cur, (char*)nxt, b->ntasks, b->d_task_s, b->d_task_cg,
-                     b->d_task_sg, b->d_aoff, b->d_lds, b->d_xoffb, b->d_active, b->d_part);
+  auto kern = b->tiling.nontemporal ? k_batch_step<T, MODE, true> : k_batch_step<T, MODE, false>;
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, b->ctx->stream, b->d_rowptr, b->d_col, hv,
+                     (const char*)cur, (char*)nxt, b->nslots, b->d_slots, b->d_task_s, b->d_task_c0,
+                     b->d_task_sg, b->d_aoff, b->d_lds, b->d_xoffb, b->d_active, b->d_part,
+                     b->tiling.atoms_per_wave);
 }
 
 template <typename T>
@@ -418,16 +569,26 @@ void launch_step(ltb_batch* b, int mode, int grid, const void* cur, void* nxt) {
 
 // the exact-trace moments of every active structure; `precs` bit 0: some
 // structure recurses in FP32, bit 1: some in FP64
-int batch_moments(ltb_batch* b, int M, int precs) {
+int batch_moments(ltb_batch* b, int M, int precs, bool capturing) {
   ltb_ctx* c = b->ctx;
   const int nsteps = (M + 1) / 2;
-  const int grid = std::min(b->ntasks, 8 * std::max(c->cu_count, 8));
+  const int grid = step_grid(b);
+  const bool timed = c->timing && !capturing;
+  if (timed) {
+    while ((int)b->events.size() < 2 * nsteps) {
+      hipEvent_t e;
+      LTB_HIP(c, hipEventCreate(&e));
+      b->events.push_back(e);
+    }
+  }
   for (int n = 0; n < nsteps; n++) {
     const int mode = n == 0 ? 0 : (n == 1 ? 1 : 2);
     void* cur = (n & 1) ? b->d_xb : b->d_xa;
     void* nxt = (n == 0) ? b->d_xb : ((n & 1) ? b->d_xa : b->d_xb);
+    if (timed) LTB_HIP(c, hipEventRecord(b->events[2 * n], c->stream));
     if (precs & 1) launch_step<float>(b, mode, grid, cur, nxt);
     if (precs & 2) launch_step<double>(b, mode, grid, cur, nxt);
+    if (timed) LTB_HIP(c, hipEventRecord(b->events[2 * n + 1], c->stream));
     hipLaunchKernelGGL(k_batch_reduce, dim3((b->ncols + 255) / 256), dim3(256), 0, c->stream, b->d_part,
                        b->ncols, b->d_gc_s, b->d_gc_c, b->d_tbegin, b->d_nsg, b->d_aoff, b->d_active, n,
                        M, b->d_mu1, b->d_mom, b->nat);
@@ -438,7 +599,7 @@ int batch_moments(ltb_batch* b, int M, int precs) {
   return LTB_OK;
 }
 
-int batch_iteration(ltb_batch* b, const ltb_scc_params* prm, int precs) {
+int batch_iteration(ltb_batch* b, const ltb_scc_params* prm, int precs, bool capturing) {
   ltb_ctx* c = b->ctx;
   const int M = prm->M, K = prm->K;
   hipLaunchKernelGGL(k_batch_shift, dim3((b->nat + 255) / 256), dim3(256), 0, c->stream, b->d_grow,
@@ -448,7 +609,7 @@ int batch_iteration(ltb_batch* b, const ltb_scc_params* prm, int precs) {
   hipLaunchKernelGGL(k_batch_rescale, dim3((b->nnzb * 16 + 255) / 256), dim3(256), 0, c->stream, b->d_h0,
                      b->d_blkrow, b->d_col, b->d_atomst, b->d_v, b->d_sc, b->d_active, b->nnzb, b->d_ht64,
                      b->d_ht32);
-  int st = batch_moments(b, M, precs);
+  int st = batch_moments(b, M, precs, capturing);
   if (st) return st;
   hipLaunchKernelGGL(k_batch_weights, dim3((K + 255) / 256, b->S), dim3(256), 0, c->stream, b->d_tot,
                      b->d_g, M, K, b->d_active, b->d_W, b->d_X);
@@ -511,25 +672,17 @@ int ltb_batch_create(ltb_ctx* c, int nstruct, const ltb_system* const* systems, 
     }
     b->aoff.push_back(a0 + n);
     b->Ne.push_back(sy->nelectrons);
+    b->nnzb_s.push_back(sy->h_rowptr[n]);
   }
   b->nat = b->aoff.back();
   b->nnzb = (int)col.size();
-  // vectors, tasks, columns
-  std::vector<int> task_s, task_cg, task_sg, tbegin, nsg, gc_s, gc_c;
+  // vectors and columns (tasks depend on the tiling: build_tasks)
+  std::vector<int> gc_s, gc_c;
   b->xoff.push_back(0);
   for (int s = 0; s < nstruct; s++) {
     const int nat = b->aoff[s + 1] - b->aoff[s], norb = 4 * nat, ld = (norb + 63) / 64 * 64;
     b->lds.push_back(ld);
     b->xoff.push_back(b->xoff.back() + (size_t)norb * ld);
-    const int ng = (nat + 4 * APW - 1) / (4 * APW), ncg = ld / 64;
-    tbegin.push_back((int)task_s.size());
-    nsg.push_back(ng);
-    for (int cg = 0; cg < ncg; cg++)
-      for (int g = 0; g < ng; g++) {
-        task_s.push_back(s);
-        task_cg.push_back(cg);
-        task_sg.push_back(g);
-      }
     for (int cc = 0; cc < ld; cc++) {
       gc_s.push_back(s);
       gc_c.push_back(cc);
@@ -538,7 +691,6 @@ int ltb_batch_create(ltb_ctx* c, int nstruct, const ltb_system* const* systems, 
   b->xelems = b->xoff.back();
   std::vector<size_t> xoffb(b->xoff.begin(), b->xoff.end() - 1);
   for (size_t& o : xoffb) o *= sizeof(double);  // room for FP64 vectors; FP32 uses half
-  b->ntasks = (int)task_s.size();
   b->ncols = (int)gc_s.size();
   std::vector<int> ones(nstruct, 1);
   std::vector<double> sc((size_t)nstruct * NSC, 0.0);
@@ -548,10 +700,7 @@ int ltb_batch_create(ltb_ctx* c, int nstruct, const ltb_system* const* systems, 
       (st = bupload(b, &b->d_blkrow, blkrow)) || (st = bupload(b, &b->d_atomst, atomst)) ||
       (st = bupload(b, &b->d_grow, grow)) || (st = bupload(b, &b->d_gcol, gcol)) ||
       (st = bupload(b, &b->d_aoff, b->aoff)) || (st = bupload(b, &b->d_lds, b->lds)) ||
-      (st = bupload(b, &b->d_active, ones)) || (st = bupload(b, &b->d_task_s, task_s)) ||
-      (st = bupload(b, &b->d_task_cg, task_cg)) || (st = bupload(b, &b->d_task_sg, task_sg)) ||
-      (st = bupload(b, &b->d_tbegin, tbegin)) || (st = bupload(b, &b->d_nsg, nsg)) ||
-      (st = bupload(b, &b->d_gc_s, gc_s)) || (st = bupload(b, &b->d_gc_c, gc_c)) ||
+      (st = bupload(b, &b->d_active, ones)) || (st = bupload(b, &b->d_gc_s, gc_s)) || (st = bupload(b, &b->d_gc_c, gc_c)) ||
       (st = bupload(b, &b->d_xoffb, xoffb)) || (st = bupload(b, &b->d_h0, h0)) ||
       (st = bupload(b, &b->d_diag0, diag0)) || (st = bupload(b, &b->d_offabs, offabs)) ||
       (st = bupload(b, &b->d_gval, gval)) || (st = bupload(b, &b->d_p0, p0)) ||
@@ -559,8 +708,7 @@ int ltb_batch_create(ltb_ctx* c, int nstruct, const ltb_system* const* systems, 
       (st = bupload(b, &b->d_sc, sc)) || (st = bmalloc(b, &b->d_v, sizeof(double) * b->nat)) ||
       (st = bmalloc(b, &b->d_ht64, sizeof(double) * 16 * b->nnzb)) ||
       (st = bmalloc(b, &b->d_ht32, sizeof(float) * 16 * b->nnzb)) ||
-      (st = bmalloc(b, &b->d_part, sizeof(double) * 128 * (size_t)b->ntasks)) ||
-      (st = bmalloc(b, &b->d_mu1, sizeof(double) * b->ncols))) {
+      (st = bmalloc(b, &b->d_mu1, sizeof(double) * b->ncols)) || (st = build_tasks(b))) {
     ltb_batch_destroy(b);
     return st;
   }
@@ -576,10 +724,20 @@ int ltb_batch_sizes(const ltb_batch* b, int* nstruct, int* natoms_total, int* at
   return LTB_OK;
 }
 
+int ltb_batch_stats(const ltb_batch* b, long long* launches, double* seconds, double* bytes) {
+  if (!b) return LTB_ERR_ARG;
+  if (launches) *launches = b->stat_launches;
+  if (seconds) *seconds = b->stat_seconds;
+  if (bytes) *bytes = b->stat_bytes;
+  return LTB_OK;
+}
+
 void ltb_batch_destroy(ltb_batch* b) {
   if (!b) return;
   (void)hipStreamSynchronize(b->ctx->stream);
   for (void* p : b->allocs) (void)hipFree(p);
+  free_tasks(b);
+  for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
   for (void* p : {(void*)b->d_xa, (void*)b->d_xb, (void*)b->d_mom, (void*)b->d_tot, (void*)b->d_W,
                   (void*)b->d_X, (void*)b->d_g, (void*)b->d_w})
     if (p) (void)hipFree(p);
@@ -595,17 +753,8 @@ int ltb_batch_scc_run(ltb_batch* b, const ltb_scc_params* prm, const double* gue
     return set_err(c, LTB_ERR_ARG, "bad batch SCC parameters (Fermi-Dirac kT > 0 required)");
   const int M = prm->M, K = prm->K, S = b->S;
   auto t0 = std::chrono::steady_clock::now();
-  // buffers sized for the precision of the bulk (FP64 needs twice the FP32 room)
-  const size_t xb = b->xelems * sizeof(double);
-  if (b->xbytes < xb) {
-    for (void** p : {&b->d_xa, &b->d_xb}) {
-      if (*p) (void)hipFree(*p);
-      *p = nullptr;
-    }
-    LTB_HIP(c, hipMalloc(&b->d_xa, xb));
-    LTB_HIP(c, hipMalloc(&b->d_xb, xb));
-    b->xbytes = xb;
-  }
+  int st0 = ensure_vectors(b);
+  if (st0) return st0;
   if (b->M != M || b->K != K) {
     for (void** p : {(void**)&b->d_mom, (void**)&b->d_tot, (void**)&b->d_W, (void**)&b->d_X, (void**)&b->d_g,
                      (void**)&b->d_w}) {
@@ -633,6 +782,8 @@ int ltb_batch_scc_run(ltb_batch* b, const ltb_scc_params* prm, const double* gue
   std::vector<int> active(S, start), it_s(S, 0), st_s(S, LTB_NOT_CONVERGED);
   std::vector<double> sc((size_t)S * NSC);
   int it = 0, n64 = 0, ngraph = 0;
+  b->stat_launches = 0;
+  b->stat_seconds = b->stat_bytes = 0;
   hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};
   bool warmed[4] = {false, false, false, false};
   int result = LTB_OK;
@@ -645,7 +796,7 @@ int ltb_batch_scc_run(ltb_batch* b, const ltb_scc_params* prm, const double* gue
       if (!exec[precs]) {
         hipGraph_t g = nullptr;
         LTB_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        const int s2 = batch_iteration(b, prm, precs);
+        const int s2 = batch_iteration(b, prm, precs, true);
         const hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (s2) return s2;
         if (e != hipSuccess) return set_err(c, LTB_ERR_DEVICE, std::string("batch graph capture: ") + hipGetErrorString(e));
@@ -656,9 +807,19 @@ int ltb_batch_scc_run(ltb_batch* b, const ltb_scc_params* prm, const double* gue
       LTB_HIP(c, hipGraphLaunch(exec[precs], c->stream));
       ngraph++;
     } else {
-      const int s2 = batch_iteration(b, prm, precs);
+      const int s2 = batch_iteration(b, prm, precs, false);
       if (s2) return s2;
       warmed[precs] = true;
+      if (c->timing) {
+        LTB_HIP(c, hipStreamSynchronize(c->stream));
+        for (int n = 0; n < (M + 1) / 2; n++) {
+          float ms = 0;
+          LTB_HIP(c, hipEventElapsedTime(&ms, b->events[2 * n], b->events[2 * n + 1]));
+          b->stat_seconds += ms * 1e-3;
+        }
+        b->stat_launches += (M + 1) / 2;
+        b->stat_bytes += iteration_bytes(b, active, M);
+      }
     }
     if (precs & 2) n64++;
     LTB_HIP(c, hipMemcpyAsync(sc.data(), b->d_sc, sizeof(double) * sc.size(), hipMemcpyDeviceToHost, c->stream));
@@ -714,6 +875,103 @@ int ltb_batch_scc_run(ltb_batch* b, const ltb_scc_params* prm, const double* gue
   }
   if (result) return set_err(c, result, "non-finite value in a batched structure");
   if (nconv < S) return set_err(c, LTB_NOT_CONVERGED, std::to_string(S - nconv) + " structures not converged");
+  return LTB_OK;
+}
+
+int ltb_batch_set_tiling(ltb_batch* b, const ltb_kpe_tiling* t) {
+  if (!b || !t) return LTB_ERR_ARG;
+  if (t->atoms_per_wave < 0 || t->atoms_per_wave > 64 || t->blocks_per_cu < 0 || t->blocks_per_cu > 16)
+    return set_err(b->ctx, LTB_ERR_ARG, "batch tiling: atoms_per_wave in 1..64, blocks_per_cu in 1..16");
+  ltb_kpe_tiling n = b->tiling;
+  if (t->atoms_per_wave) n.atoms_per_wave = t->atoms_per_wave;
+  if (t->blocks_per_cu) n.blocks_per_cu = t->blocks_per_cu;
+  if (t->nontemporal >= 0) n.nontemporal = t->nontemporal ? 1 : 0;
+  b->tiling = n;
+  return build_tasks(b);
+}
+
+int ltb_batch_get_tiling(const ltb_batch* b, ltb_kpe_tiling* out) {
+  if (!b || !out) return LTB_ERR_ARG;
+  *out = b->tiling;
+  return LTB_OK;
+}
+
+int ltb_batch_autotune(ltb_batch* b, int precision, int trial_steps, ltb_kpe_tiling* best,
+                       double* best_gbs) {
+  if (!b || trial_steps < 1 || (precision != LTB_FP64 && precision != LTB_FP32_ACC64)) return LTB_ERR_ARG;
+  ltb_ctx* c = b->ctx;
+  int st = ensure_vectors(b);
+  if (st) return st;
+  // H~ of the neutral start for every structure
+  std::vector<int> active(b->S, precision == LTB_FP64 ? 2 : 1);
+  LTB_HIP(c, hipMemcpy(b->d_active, active.data(), sizeof(int) * b->S, hipMemcpyHostToDevice));
+  LTB_HIP(c, hipMemcpy(b->d_pin, b->d_p0, sizeof(double) * b->nat, hipMemcpyDeviceToDevice));
+  hipLaunchKernelGGL(k_batch_shift, dim3((b->nat + 255) / 256), dim3(256), 0, c->stream, b->d_grow,
+                     b->d_gcol, b->d_gval, b->d_pin, b->d_p0, b->d_atomst, b->d_active, b->d_v, b->nat);
+  hipLaunchKernelGGL(k_batch_gersh, dim3(b->S), dim3(256), 0, c->stream, b->d_diag0, b->d_offabs, b->d_v,
+                     b->d_aoff, b->d_active, b->d_sc);
+  hipLaunchKernelGGL(k_batch_rescale, dim3((b->nnzb * 16 + 255) / 256), dim3(256), 0, c->stream, b->d_h0,
+                     b->d_blkrow, b->d_col, b->d_atomst, b->d_v, b->d_sc, b->d_active, b->nnzb, b->d_ht64,
+                     b->d_ht32);
+  LTB_HIP(c, hipGetLastError());
+  // bytes of one step n >= 2 over all structures
+  double step_bytes = 0;
+  const double el = precision == LTB_FP64 ? 8.0 : 4.0;
+  for (int s = 0; s < b->S; s++) {
+    const double norb = 4.0 * (b->aoff[s + 1] - b->aoff[s]);
+    step_bytes += 3 * el * norb * norb + b->nnzb_s[s] * (16.0 * el + 4.0) + 4.0 * (norb / 4 + 1);
+  }
+  hipEvent_t e0, e1;
+  LTB_HIP(c, hipEventCreate(&e0));
+  LTB_HIP(c, hipEventCreate(&e1));
+  const ltb_kpe_tiling keep = b->tiling;
+  ltb_kpe_tiling top = keep;
+  double top_gbs = -1;
+  for (int apw : {4, 8, 16})
+    for (int bpc : {2, 4, 8})
+      for (int nt : {0, 1}) {
+        b->tiling.atoms_per_wave = apw;
+        b->tiling.blocks_per_cu = bpc;
+        b->tiling.nontemporal = nt;
+        if ((st = build_tasks(b))) break;
+        const int grid = step_grid(b);
+        auto step = [&](int n) {
+          const int mode = n == 0 ? 0 : (n == 1 ? 1 : 2);
+          void* cur = (n & 1) ? b->d_xb : b->d_xa;
+          void* nxt = (n == 0) ? b->d_xb : ((n & 1) ? b->d_xa : b->d_xb);
+          if (precision == LTB_FP64)
+            launch_step<double>(b, mode, grid, cur, nxt);
+          else
+            launch_step<float>(b, mode, grid, cur, nxt);
+        };
+        step(0);
+        step(1);
+        (void)hipEventRecord(e0, c->stream);
+        for (int n = 2; n < 2 + trial_steps; n++) step(n);
+        (void)hipEventRecord(e1, c->stream);
+        if (hipStreamSynchronize(c->stream) != hipSuccess || hipGetLastError() != hipSuccess) {
+          st = set_err(c, LTB_ERR_DEVICE, "batch autotune trial failed");
+          break;
+        }
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        const double gbs = step_bytes * trial_steps / (ms * 1e-3) / 1e9;
+        if (gbs > top_gbs) {
+          top_gbs = gbs;
+          top = b->tiling;
+        }
+      }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st) {
+    b->tiling = keep;
+    (void)build_tasks(b);
+    return st;
+  }
+  b->tiling = top;
+  if ((st = build_tasks(b))) return st;
+  if (best) *best = top;
+  if (best_gbs) *best_gbs = top_gbs;
   return LTB_OK;
 }
 

@@ -135,3 +135,21 @@ def test_batch_of_uneven_training_cells(ctx):
         ctx.load(systems[k])
         g = ctx.scc(256, precision=L.FP64, tol=1e-6, max_iter=400, use_graph=1)
         assert np.abs(qs[k] - g["q"]).max() < 1e-6, k
+
+
+def test_batch_tilings_agree_and_autotune(batch):
+    """Every tiling of the batched step kernel gives the same FP64 SCC (the
+    grouping of the FP64 partial sums differs, so 1e-10 e), and the autotuner
+    leaves a valid tiling behind."""
+    base = batch.scc(200, precision=L.FP64, tol=1e-6, max_iter=300)
+    for apw, bpc, nt in [(4, 2, 0), (16, 8, 1), (1, 1, 1)]:
+        batch.set_tiling(atoms_per_wave=apw, blocks_per_cu=bpc, nontemporal=nt)
+        t = batch.tiling()
+        assert (t["atoms_per_wave"], t["blocks_per_cu"], t["nontemporal"]) == (apw, bpc, nt)
+        r = batch.scc(200, precision=L.FP64, tol=1e-6, max_iter=300)
+        assert np.array_equal(r["iters"], base["iters"])
+        assert np.abs(r["q"] - base["q"]).max() < 1e-10
+    best, gbs = batch.autotune(L.FP32_ACC64, 2)
+    assert gbs > 0 and best["atoms_per_wave"] in (4, 8, 16) and best["blocks_per_cu"] in (2, 4, 8)
+    r = batch.scc(200, precision=L.FP64, tol=1e-6, max_iter=300)
+    assert np.abs(r["q"] - base["q"]).max() < 1e-10
