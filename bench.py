@@ -70,8 +70,9 @@ def measured_traffic(cells, M, tiling, prec_name):
     path = os.path.join(ROOT, "profiles", "cheb_sweep_fp32_quartz12_M512.txt")
     if not (cells == [12, 12, 12] and prec_name == "fp32" and tiling and os.path.exists(path)):
         return None
-    want = (4096, 4, 2, 2, 1)
-    got = tuple(tiling[k] for k in ("chunk_cols", "cols_per_lane", "schedule", "blocks_per_cu", "nontemporal"))
+    want = (4096, 4, 2, 2, 1, 2)
+    got = tuple(tiling[k] for k in ("chunk_cols", "cols_per_lane", "schedule", "blocks_per_cu", "nontemporal",
+                                    "order"))
     if got != want:
         return None
     m = re.search(r"= ([0-9.]+) GB per launch", open(path).read())
@@ -200,16 +201,19 @@ def batch_measure(args):
     t0 = time.perf_counter()
     tiling, tune_gbs = b.autotune(L.FP32_ACC64, 3)
     log(f"batch autotune {time.perf_counter() - t0:.2f} s: {tiling} at {tune_gbs:.0f} GB/s")
-    kw = dict(precision=L.FP32_ACC64, switch_tol=1e-6, tol=1e-6, max_iter=400)
+    kw = dict(precision=L.FP32_ACC64, switch_tol=1e-6, tol=1e-6, max_iter=args.max_iter, check=False)
     b.scc(M, max_iter=1, tol=1e-30, check=False)                       # warm-up, FP32 path
     b.scc(M, precision=L.FP64, max_iter=1, tol=1e-30, check=False)     # and FP64 path
     ctx.set_timing(True)
     eager = b.scc(M, use_graph=0, **kw)
     st = b.stats()
     ctx.set_timing(False)
-    log(f"eager: {eager['batch_iters']} batch iterations, {eager['seconds']:.2f} s")
-    graph = b.scc(M, use_graph=1, verbose=1, **kw)
-    log(f"graph: {graph['batch_iters']} batch iterations, {graph['seconds']:.2f} s")
+    log(f"eager: {eager['batch_iters']} batch iterations, {eager['seconds']:.2f} s, status {eager['status']}")
+    if args.no_graph:
+        graph = eager
+    else:
+        graph = b.scc(M, use_graph=1, verbose=1, **kw)
+        log(f"graph: {graph['batch_iters']} batch iterations, {graph['seconds']:.2f} s")
     qb = b.split(graph["q"])
     b.close()
     # the same cells one at a time (single-structure SCC, same settings, graphs)
@@ -221,7 +225,7 @@ def batch_measure(args):
         t1 = time.perf_counter()
         r = ctx.scc(M, use_graph=1, **kw)
         tseq += time.perf_counter() - t1
-        ref = ctx.scc(M, precision=L.FP64, tol=1e-6, max_iter=400, use_graph=1)
+        ref = ctx.scc(M, precision=L.FP64, tol=1e-6, max_iter=args.max_iter, use_graph=1, check=False)
         err = max(err, float(np.abs(qb[k] - ref["q"]).max()))
         eerr = max(eerr, abs(graph["eband_fp"][k] - ref["eband_fp"]) * 27.211386 / systems[k].nat)
     ctx.close()
@@ -230,9 +234,11 @@ def batch_measure(args):
     out = {"workload": f"batched SCC, {len(systems)} uneven alpha-quartz cells (2x2x2..5x5x4), "
                        f"{nat} atoms, exact-trace KPE M={M}, FP32 bulk + FP64 finish to 1e-6 e",
            "cells": len(systems), "atoms": nat, "M": M, "seed": args.seed,
+           "status": graph["status"],
            "batch_iterations": graph["batch_iters"], "fp64_batch_iterations": graph["fp64_iters"],
            "cell_iterations_min_max": [int(graph["iters"].min()), int(graph["iters"].max())],
-           "seconds_graph": round(graph["seconds"], 3), "seconds_eager": round(eager["seconds"], 3),
+           "seconds_graph": None if args.no_graph else round(graph["seconds"], 3),
+           "seconds_eager": round(eager["seconds"], 3),
            "cells_per_s": round(len(systems) / graph["seconds"], 2),
            "one_at_a_time_seconds": round(seq_all, 3),
            "one_at_a_time_sample": f"{len(pick)} of {len(systems)} cells, scaled to all",
@@ -245,6 +251,72 @@ def batch_measure(args):
                         "mean_launch_us": round(st["seconds"] / max(st["launches"], 1) * 1e6, 2),
                         "tiling": {k: tiling[k] for k in ("blocks_per_cu", "atoms_per_wave", "nontemporal")}}}
     print(json.dumps(out))
+
+
+def scc_segment(args):
+    """One segment of a long SCC solve, resumable: linear mixing keeps no state
+    but the mixed populations, so a run cut into segments (each under one
+    process) repeats the uninterrupted iteration sequence.  The state file
+    carries the populations, iteration counts and the summed loop seconds."""
+    import numpy as np
+    import largetb_amd as L
+    from largetb_amd import learned
+    s = L.System.quartz(*args.cells, sigma_ang=args.sigma, seed=args.seed)
+    ctx = L.Context(0)
+    ctx.load(s)
+    st = dict(np.load(args.state, allow_pickle=False)) if args.state and os.path.exists(args.state) else None
+    mixed = args.segment == "mixed"
+    prec = L.FP32_ACC64 if mixed else L.FP64
+    if st is not None and int(st["fp64_iters"]) > 0:
+        prec = L.FP64  # a mixed run that already switched stays in FP64
+    ctx.set_populations(None)
+    lo, hi = ctx.gershgorin()
+    ctx.rescale((hi - lo) / 2, (hi + lo) / 2)
+    tiling, gbs = ctx.autotune(prec, 3)
+    log(f"autotune {tiling} {gbs:.0f} GB/s")
+    ctx.scc(args.M, precision=prec, max_iter=1, tol=1e-30, check=False)  # warm-up, not counted
+    extra = 0.0
+    if st is not None:
+        guess = st["pop"]
+    elif args.start == "learned":
+        m, info = learned.load(ctx)
+        t0 = time.perf_counter()
+        guess = m.predict(s)
+        extra = time.perf_counter() - t0
+        m.close()
+    else:
+        guess = None
+    r = ctx.scc(args.M, guess=guess, precision=prec, switch_tol=1e-6 if mixed else 0.0, tol=1e-6,
+                max_iter=args.max_iter, use_graph=1, verbose=1, check=False)
+    prev = st or dict(iters=0, fp64_iters=0, seconds=0.0, hist=np.zeros(0))
+    out = dict(pop=ctx.mixed_populations(), iters=int(prev["iters"]) + r["iters"],
+               fp64_iters=int(prev["fp64_iters"]) + (r["fp64_iters"] if mixed else r["iters"]),
+               seconds=float(prev["seconds"]) + r["seconds"] + extra,
+               hist=np.concatenate([np.asarray(prev["hist"]), r["hist"]]), status=r["status"],
+               q=r["q"], eband_fp=r["eband_fp"], mu=r["mu_ha"], nat=s.nat)
+    np.savez(args.state_out, **out)
+    summary = {k: (v if not isinstance(v, np.ndarray) else None) for k, v in out.items()
+               if k not in ("pop", "q", "hist")}
+    summary.update(workload=f"{s.nat}-atom alpha-quartz {'x'.join(map(str, args.cells))}, M={args.M}",
+                   run=f"{args.start} start, {args.segment}", segment_iters=r["iters"],
+                   segment_seconds=round(r["seconds"], 3), last_dq=float(r["dq"]))
+    print(json.dumps(summary))
+    ctx.close()
+
+
+def compare_states(paths):
+    """Errors of a finished run (second file) against the plain FP64 one (first)."""
+    import numpy as np
+    a, b = (dict(np.load(p, allow_pickle=False)) for p in paths)
+    nat = int(a["nat"])
+    print(json.dumps({
+        "plain": {"iters": int(a["iters"]), "seconds": round(float(a["seconds"]), 2), "status": int(a["status"])},
+        "fast": {"iters": int(b["iters"]), "fp64_iters": int(b["fp64_iters"]),
+                 "seconds": round(float(b["seconds"]), 2), "status": int(b["status"])},
+        "speedup": round(float(a["seconds"]) / float(b["seconds"]), 3),
+        "max_dq_e": float(np.abs(a["q"] - b["q"]).max()),
+        "deband_ev_per_atom": abs(float(a["eband_fp"]) - float(b["eband_fp"])) * 27.211386 / nat,
+        "dmu_ev": abs(float(a["mu"]) - float(b["mu"])) * 27.211386}))
 
 
 def main():
@@ -260,20 +332,36 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tiling", type=str, default="",
-                    help="explicit tiling (implies --no-autotune): chunk,V,schedule,blocks_per_cu,nt")
+                    help="explicit tiling (implies --no-autotune): chunk,V,schedule,blocks_per_cu,nt[,order]")
     ap.add_argument("--no-ablation", action="store_true")
     ap.add_argument("--sp2", action="store_true", help="dense SP2 measurement (8x8x8 by default)")
     ap.add_argument("--tts", action="store_true",
                     help="SCC time-to-solution: plain FP64 neutral start vs learned guess + mixed")
     ap.add_argument("--batch", type=int, default=0,
                     help="batched SCC over this many uneven cells (M=256 unless --M is given)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batch mode: eager iterations only (for kernel tracing)")
     ap.add_argument("--batch-seq", type=int, default=32,
                     help="cells re-solved one at a time for the comparison (0: all)")
+    ap.add_argument("--segment", choices=["plain", "mixed"],
+                    help="one resumable segment of an SCC solve (plain FP64 or FP32 bulk + FP64 finish)")
+    ap.add_argument("--start", choices=["neutral", "learned"], default="neutral")
+    ap.add_argument("--state", default="", help="segment: state file to resume from")
+    ap.add_argument("--state-out", default="gpurun_out/scc_state.npz")
+    ap.add_argument("--max-iter", type=int, default=400)
+    ap.add_argument("--compare", nargs=2, metavar=("PLAIN", "FAST"),
+                    help="errors and speed-up between two finished segment state files")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 1 GiB device-to-device copies (counter calibration)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.compare:
+        compare_states(args.compare)
+        return
+    if args.segment:
+        scc_segment(args)
         return
     if args.tts:
         time_to_solution(args)
@@ -298,8 +386,11 @@ def main():
     ctx.load(s)
     tiling = None
     if args.tiling:
-        ch, v, sch, bpc, nt = (int(x) for x in args.tiling.split(","))
-        ctx.set_tiling(chunk_cols=ch, cols_per_lane=v, schedule=sch, blocks_per_cu=bpc, nontemporal=nt)
+        vals = [int(x) for x in args.tiling.split(",")]
+        ch, v, sch, bpc, nt = vals[:5]
+        order = vals[5] if len(vals) > 5 else 0
+        ctx.set_tiling(chunk_cols=ch, cols_per_lane=v, schedule=sch, blocks_per_cu=bpc, nontemporal=nt,
+                       order=order)
         tiling = ctx.get_tiling(prec)
     elif not args.no_autotune:
         ctx.set_populations(None)

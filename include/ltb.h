@@ -76,6 +76,25 @@ int ltb_system_gamma(const ltb_system* sys, int* rowptr, int* col, double* val);
  * the O branch the reference lacks). */
 int ltb_system_neutral(const ltb_system* sys, double* p0);
 
+/* ---- MATLAB ASCII files (the reference's dump format) ---------------------------
+ * The layout of PETSc's PETSC_VIEWER_ASCII_MATLAB viewer, in which the
+ * reference writes GAMMA_AB.m / orbit2Atom.m (:950-963), deltaH.m, netCharge.m,
+ * GammaQ_vector.m (:1465-1484) and H.m, Htotal_scc.m, Htotal0.m (:1504-1523):
+ * 1-based "i j  v" rows of the scalar entries (row-major, ascending columns),
+ * then NAME = spconvert(zzz); vectors one value per row.  Values as %.16e.
+ * A BSR matrix with block size bs is written entry by entry (bs = 1: CSR);
+ * nrows / ncols count block rows / block columns.  name: a MATLAB identifier. */
+int ltb_write_matlab_bsr(const char* path, const char* name, int nrows, int ncols, int bs, const int* rowptr,
+                         const int* col, const double* val);
+int ltb_write_matlab_vec(const char* path, const char* name, int n, const double* v);
+/* Readers for the same files (also the reference's own dumps).  *nnz / *n:
+ * capacity in, count out; with NULL outputs they only report sizes.  Indices
+ * come back 0-based. */
+int ltb_read_matlab_mat(const char* path, long long* nrows, long long* ncols, long long* nnz, int* rows,
+                        int* cols, double* vals);
+int ltb_read_matlab_vec(const char* path, long long* n, double* v);
+const char* ltb_io_last_error(void); /* thread-local message of the last file error */
+
 /* ---- device context ---------------------------------------------------------- */
 int ltb_create(int device, ltb_ctx** out);
 void ltb_destroy(ltb_ctx* ctx);
@@ -83,6 +102,22 @@ const char* ltb_last_error(const ltb_ctx* ctx);
 
 /* Upload H0, Gamma and the orbital->atom map (initializeForH + MatAssembly). */
 int ltb_load_system(ltb_ctx* ctx, const ltb_system* sys);
+
+/* The same system assembled on the device from coordinates (Bohr; Z 14 Si /
+ * 8 O; cell9 rows = periodic vectors, may be NULL without periodicity):
+ * O(N) cell-list neighbour search, 4x4 Slater-Koster blocks, Gamma and the
+ * Gershgorin sums in device kernels.  Replaces getHaloAtoms (:372-501, an
+ * O(N^2) host scan), get4by4HmatrixElements (:211-349), constructH
+ * (:969-1066) and initializeForSCF's Gamma (:854-900); the blocks, pattern
+ * and order equal ltb_system_create's.  Periodic rows must be wider than
+ * 8 Bohr (one image per pair).  seconds (optional): wall time of the assembly. */
+int ltb_load_coords(ltb_ctx* ctx, int natoms, const int* Z, const double* xyz_bohr, const double* cell9,
+                    int periodic_mask, double* seconds);
+/* sizes and a host copy of the loaded system's H0 (BSR, as
+ * ltb_system_hamiltonian) and Gamma (CSR, as ltb_system_gamma) */
+int ltb_get_sizes(ltb_ctx* ctx, int* natoms, int* nnzb, int* gamma_nnz, double* nelectrons);
+int ltb_get_hamiltonian(ltb_ctx* ctx, int* rowptr, int* col, double* val16);
+int ltb_get_gamma(ltb_ctx* ctx, int* rowptr, int* col, double* val);
 
 /* constructdeltaH (:1414-1487): v = Gamma (p - p0) on device (FP64), H_total =
  * H0 + diag(v repeated over the 4 orbitals).  p: per-spin populations [natoms]. */
@@ -168,6 +203,12 @@ typedef struct {
  * hist (optional, [max_iter]) receives |dq| per iteration. */
 int ltb_scc_run(ltb_ctx* ctx, const ltb_scc_params* prm, const double* p_guess, double* p_out,
                 double* hist, ltb_scc_result* res);
+/* The per-spin populations the next SCC iteration would start from: q_in / 2
+ * after the last mixing step of ltb_scc_run (the reference's mixed charges,
+ * updateHtotal :1489-1526 in charge form).  Passing them as p_guess to a new
+ * ltb_scc_run continues the same iteration sequence exactly (linear mixing
+ * keeps no other state), so a long SCC can run in segments. */
+int ltb_get_mixed_populations(ltb_ctx* ctx, double* p);
 
 /* ---- many independent structures in one SCC ----------------------------------
  * The reference solves one structure per job (main :1742-1849, testjob2.pbs);
@@ -248,6 +289,8 @@ typedef struct {
   int blocks_per_cu;  /* sweep: resident workgroups per CU (0 = auto)            */
   int atoms_per_wave; /* blocked: consecutive atoms per wave (0 = auto)          */
   int nontemporal;    /* 1: t_{n-1}, t_{n+1} bypass the caches; 0: cached; -1 auto */
+  int order;          /* sweep visiting order of the atoms: 0 auto (= 2), 1 spatial
+                         strips (bonded atoms close in sequence), 2 atom index   */
 } ltb_kpe_tiling;
 int ltb_set_kpe_tiling(ltb_ctx* ctx, const ltb_kpe_tiling* t);
 /* the tiling the kernel would use for the loaded system (auto fields resolved) */

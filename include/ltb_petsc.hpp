@@ -42,6 +42,8 @@ struct Settings {
   int max_scf = 20;                     // :1820
   int device = 0;
   bool verbose = true;
+  std::string dump_dir;                 // non-empty: write the reference's MATLAB dumps there (H.m,
+                                        // GAMMA_AB.m, deltaH.m, netCharge.m, GammaQ_vector.m)
 };
 
 struct State {

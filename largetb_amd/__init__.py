@@ -52,7 +52,8 @@ class DensityResult(C.Structure):
 
 class KpeTiling(C.Structure):
     _fields_ = [("chunk_cols", C.c_int), ("cols_per_lane", C.c_int), ("schedule", C.c_int),
-                ("blocks_per_cu", C.c_int), ("atoms_per_wave", C.c_int), ("nontemporal", C.c_int)]
+                ("blocks_per_cu", C.c_int), ("atoms_per_wave", C.c_int), ("nontemporal", C.c_int),
+                ("order", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -93,12 +94,23 @@ SIGNATURES = {
     "ltb_system_hamiltonian": (C.c_int, [VP, PI, PI, PD]),
     "ltb_system_gamma": (C.c_int, [VP, PI, PI, PD]),
     "ltb_system_neutral": (C.c_int, [VP, PD]),
+    "ltb_write_matlab_bsr": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, PI, PI, PD]),
+    "ltb_write_matlab_vec": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, PD]),
+    "ltb_read_matlab_mat": (C.c_int, [C.c_char_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                      C.POINTER(C.c_longlong), PI, PI, PD]),
+    "ltb_read_matlab_vec": (C.c_int, [C.c_char_p, C.POINTER(C.c_longlong), PD]),
+    "ltb_io_last_error": (C.c_char_p, []),
     "ltb_create": (C.c_int, [C.c_int, C.POINTER(VP)]),
     "ltb_destroy": (None, [VP]),
     "ltb_last_error": (C.c_char_p, [VP]),
     "ltb_load_system": (C.c_int, [VP, VP]),
+    "ltb_load_coords": (C.c_int, [VP, C.c_int, PI, PD, PD, C.c_int, PD]),
+    "ltb_get_sizes": (C.c_int, [VP, PI, PI, PI, PD]),
+    "ltb_get_hamiltonian": (C.c_int, [VP, PI, PI, PD]),
+    "ltb_get_gamma": (C.c_int, [VP, PI, PI, PD]),
     "ltb_set_populations": (C.c_int, [VP, PD]),
     "ltb_get_shift": (C.c_int, [VP, PD]),
+    "ltb_get_mixed_populations": (C.c_int, [VP, PD]),
     "ltb_gershgorin": (C.c_int, [VP, PD, PD]),
     "ltb_rescale": (C.c_int, [VP, C.c_double, C.c_double]),
     "ltb_kpe_moments": (C.c_int, [VP, C.c_int, C.c_int, PD, PD]),
@@ -240,6 +252,48 @@ class System:
         return H
 
 
+def _io_check(st):
+    if st != LTB_OK:
+        raise LtbError(st, lib().ltb_io_last_error().decode())
+
+
+def write_matlab_bsr(path, name, rowptr, col, val, bs, ncols=None):
+    """Write a BSR (bs > 1) or CSR (bs = 1) matrix in the reference's MATLAB
+    dump layout; ncols in block columns (default: square)."""
+    rowptr = np.ascontiguousarray(rowptr, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    val = np.ascontiguousarray(val, np.float64).reshape(-1)
+    n = len(rowptr) - 1
+    _io_check(lib().ltb_write_matlab_bsr(str(path).encode(), name.encode(), n, n if ncols is None else ncols,
+                                         bs, _i(rowptr), _i(col), _d(val)))
+
+
+def write_matlab_vec(path, name, v):
+    v = np.ascontiguousarray(v, np.float64)
+    _io_check(lib().ltb_write_matlab_vec(str(path).encode(), name.encode(), len(v), _d(v)))
+
+
+def read_matlab_mat(path):
+    """(shape, rows, cols, vals) of a MATLAB spconvert dump, 0-based, file order."""
+    nr, nc, nnz = C.c_longlong(), C.c_longlong(), C.c_longlong(0)
+    _io_check(lib().ltb_read_matlab_mat(str(path).encode(), C.byref(nr), C.byref(nc), C.byref(nnz), None, None,
+                                        None))
+    r = np.zeros(nnz.value, np.int32)
+    c = np.zeros(nnz.value, np.int32)
+    v = np.zeros(nnz.value)
+    _io_check(lib().ltb_read_matlab_mat(str(path).encode(), C.byref(nr), C.byref(nc), C.byref(nnz), _i(r), _i(c),
+                                        _d(v)))
+    return (nr.value, nc.value), r, c, v
+
+
+def read_matlab_vec(path):
+    n = C.c_longlong(0)
+    _io_check(lib().ltb_read_matlab_vec(str(path).encode(), C.byref(n), None))
+    v = np.zeros(n.value)
+    _io_check(lib().ltb_read_matlab_vec(str(path).encode(), C.byref(n), _d(v)))
+    return v
+
+
 class Context:
     """Device context on one GPU (one HIP stream)."""
 
@@ -277,6 +331,37 @@ class Context:
         self.norb = system.norb
         return self
 
+    def load_coords(self, Z, xyz_bohr, cell9=None, periodic_mask=7):
+        """Assemble neighbour list, H0 and Gamma on the device from coordinates."""
+        Z = np.ascontiguousarray(Z, np.int32)
+        xyz = np.ascontiguousarray(xyz_bohr, np.float64).reshape(-1)
+        cell = None if cell9 is None else _d(np.ascontiguousarray(cell9, np.float64).reshape(-1))
+        sec = C.c_double()
+        self._check(lib().ltb_load_coords(self._h, len(Z), _i(Z), _d(xyz), cell, periodic_mask,
+                                          C.byref(sec)))
+        nat, nnzb, gnnz, ne = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        self._check(lib().ltb_get_sizes(self._h, C.byref(nat), C.byref(nnzb), C.byref(gnnz), C.byref(ne)))
+        self.system = None
+        self.nat, self.norb, self.nnzb, self.nelectrons = nat.value, 4 * nat.value, nnzb.value, ne.value
+        return sec.value
+
+    def hamiltonian(self):
+        """Host copy of the loaded H0: rowptr, col, val [nnzb, 4, 4]."""
+        rp = np.zeros(self.nat + 1, np.int32)
+        col = np.zeros(self.nnzb, np.int32)
+        val = np.zeros(16 * self.nnzb)
+        self._check(lib().ltb_get_hamiltonian(self._h, _i(rp), _i(col), _d(val)))
+        return rp, col, val.reshape(-1, 4, 4)
+
+    def gamma(self):
+        nat, nnzb, gnnz, ne = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        self._check(lib().ltb_get_sizes(self._h, C.byref(nat), C.byref(nnzb), C.byref(gnnz), C.byref(ne)))
+        rp = np.zeros(self.nat + 1, np.int32)
+        col = np.zeros(gnnz.value, np.int32)
+        val = np.zeros(gnnz.value)
+        self._check(lib().ltb_get_gamma(self._h, _i(rp), _i(col), _d(val)))
+        return rp, col, val
+
     def set_populations(self, p=None):
         pp = _d(np.ascontiguousarray(p, np.float64)) if p is not None else None
         self._check(lib().ltb_set_populations(self._h, pp))
@@ -285,6 +370,12 @@ class Context:
         v = np.zeros(self.nat)
         self._check(lib().ltb_get_shift(self._h, _d(v)))
         return v
+
+    def mixed_populations(self):
+        """Per-spin populations the next SCC iteration starts from."""
+        p = np.zeros(self.nat)
+        self._check(lib().ltb_get_mixed_populations(self._h, _d(p)))
+        return p
 
     def gershgorin(self):
         lo, hi = C.c_double(), C.c_double()
@@ -365,8 +456,8 @@ class Context:
         self._check(lib().ltb_set_kpe_timing(self._h, 1 if on else 0))
 
     def set_tiling(self, chunk_cols=0, cols_per_lane=0, schedule=SCHED_AUTO, blocks_per_cu=0,
-                   atoms_per_wave=0, nontemporal=-1):
-        t = KpeTiling(chunk_cols, cols_per_lane, schedule, blocks_per_cu, atoms_per_wave, nontemporal)
+                   atoms_per_wave=0, nontemporal=-1, order=0):
+        t = KpeTiling(chunk_cols, cols_per_lane, schedule, blocks_per_cu, atoms_per_wave, nontemporal, order)
         self._check(lib().ltb_set_kpe_tiling(self._h, C.byref(t)))
 
     def get_tiling(self, precision=FP32_ACC64):
@@ -446,7 +537,7 @@ class Batch:
         return {k: getattr(t, k) for k, _ in KpeTiling._fields_}
 
     def set_tiling(self, atoms_per_wave=0, blocks_per_cu=0, nontemporal=-1):
-        t = KpeTiling(0, 0, 0, blocks_per_cu, atoms_per_wave, nontemporal)
+        t = KpeTiling(0, 0, 0, blocks_per_cu, atoms_per_wave, nontemporal, 0)
         self.ctx._check(lib().ltb_batch_set_tiling(self._h, C.byref(t)))
 
     def autotune(self, precision=FP32_ACC64, trial_steps=3):

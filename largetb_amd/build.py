@@ -13,8 +13,8 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libltb.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["ltb_capi.hip", "cheb.hip", "sp2.hip", "apply.hip", "mlp.hip", "batch.hip"]
-CPP_SOURCES = ["system.cpp", "petsc_lite.cpp"]
+HIP_SOURCES = ["ltb_capi.hip", "cheb.hip", "sp2.hip", "apply.hip", "mlp.hip", "batch.hip", "assemble.hip"]
+CPP_SOURCES = ["system.cpp", "petsc_lite.cpp", "matlab_io.cpp"]
 DRIVER = os.path.join(HERE, "ltb_scf")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

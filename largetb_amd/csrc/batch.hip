@@ -400,7 +400,7 @@ struct ltb_batch {
   std::vector<void*> allocs;
   int *d_rowptr, *d_col, *d_blkrow, *d_atomst, *d_grow, *d_gcol, *d_aoff, *d_lds, *d_active;
   int nslots = 0;
-  ltb_kpe_tiling tiling{256, 4, LTB_SCHED_SWEEP, 8, 8, 1};  // column group, V, -, blocks/CU, atoms/wave, NT
+  ltb_kpe_tiling tiling{256, 4, LTB_SCHED_SWEEP, 8, 8, 1, 2};  // column group, V, -, blocks/CU, atoms/wave, NT, index order
   int *d_slots = nullptr, *d_task_s = nullptr, *d_task_c0 = nullptr, *d_task_sg = nullptr,
       *d_tbegin = nullptr, *d_nsg = nullptr;
   int *d_gc_s, *d_gc_c;

@@ -78,6 +78,7 @@ __global__ __launch_bounds__(256) void k_cheb_sweep(const int* __restrict__ rowp
                                                     const T* __restrict__ xcur,
                                                     T* __restrict__ xnext, int nat, int ld, int j0,
                                                     int nvalid, int n_cg, int slots,
+                                                    const int* __restrict__ visit,
                                                     double* __restrict__ part) {
   constexpr int W = 64 * V;
   const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
@@ -90,8 +91,9 @@ __global__ __launch_bounds__(256) void k_cheb_sweep(const int* __restrict__ rowp
     T sxy[V], syy[V];
 #pragma unroll
     for (int v = 0; v < V; v++) sxy[v] = syy[v] = T(0);
-    for (int A = wid; A < nat; A += S)
-      atom_step<T, V, MODE, NT>(rowptr, col, hv, xcur, xnext, A, ld, j0, c0, nvalid, sxy, syy);
+    for (int i = wid; i < nat; i += S)
+      atom_step<T, V, MODE, NT>(rowptr, col, hv, xcur, xnext, visit ? visit[i] : i, ld, j0, c0, nvalid,
+                                sxy, syy);
     flush_partials<T, V>(sxy, syy, wave, lane, part, (size_t)(xcd * slots + slot) * 2, ld, cg * W);
   }
 }
@@ -167,7 +169,7 @@ __global__ __launch_bounds__(256) void k_mom_total(const double* __restrict__ mo
 }
 
 struct Plan {
-  int V, W, ld, schedule, nt;
+  int V, W, ld, schedule, nt, order;
   int apw, n_sg;   // blocked
   int slots;       // sweep
   size_t part_rows;
@@ -189,6 +191,7 @@ Plan make_plan(const ltb_ctx* c) {
   if (p.schedule == LTB_SCHED_AUTO) p.schedule = (n_cg >= 16 && big) ? LTB_SCHED_SWEEP : LTB_SCHED_BLOCKED;
   if (p.schedule == LTB_SCHED_SWEEP && n_cg < 8) p.schedule = LTB_SCHED_BLOCKED;
   p.nt = t.nontemporal < 0 ? 1 : t.nontemporal;
+  p.order = t.order == 1 ? 1 : 2;
   const int cus = std::max(c->cu_count, 8);
   if (p.schedule == LTB_SCHED_SWEEP) {
     int bpc = t.blocks_per_cu > 0 ? t.blocks_per_cu : 4;
@@ -215,7 +218,7 @@ void launch_one(const ltb_ctx* c, const T* hv, const T* xcur, T* xnext, int ldc,
   if (p.schedule == LTB_SCHED_SWEEP) {
     hipLaunchKernelGGL((k_cheb_sweep<T, V, MODE, NT>), dim3(8 * p.slots), dim3(256), 0, c->stream,
                        c->d_rowptr, c->d_col, hv, xcur, xnext, c->nat, ldc, j0, nvalid, n_cg,
-                       p.slots, c->d_part);
+                       p.slots, p.order == 1 ? c->d_visit : nullptr, c->d_part);
   } else {
     hipLaunchKernelGGL((k_cheb_blocked<T, V, MODE, NT>), dim3(n_cg * p.n_sg), dim3(256), 0,
                        c->stream, c->d_rowptr, c->d_col, hv, xcur, xnext, c->nat, ldc, j0, nvalid,
@@ -404,5 +407,6 @@ int kpe_resolved_tiling(ltb_ctx* c, int precision, ltb_kpe_tiling* out) {
   out->blocks_per_cu = p.schedule == LTB_SCHED_SWEEP ? p.slots * 8 / std::max(c->cu_count, 8) : 0;
   out->atoms_per_wave = p.schedule == LTB_SCHED_BLOCKED ? p.apw : 0;
   out->nontemporal = p.nt;
+  out->order = p.schedule == LTB_SCHED_SWEEP ? p.order : 2;
   return LTB_OK;
 }

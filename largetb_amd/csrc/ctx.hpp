@@ -45,6 +45,7 @@ struct ltb_ctx {
   double* d_part = nullptr;
   size_t partbytes = 0;
   double* d_mu1 = nullptr;      // per column [T_1]_jj
+  int* d_visit = nullptr;       // sweep visiting order (ltb::locality_order)
   size_t mu1bytes = 0;
   double* d_mom = nullptr;      // [M+1][nat]
   double* d_momtot = nullptr;   // [M+1]
@@ -58,7 +59,7 @@ struct ltb_ctx {
   double *d_W = nullptr, *d_X = nullptr, *d_g = nullptr, *d_c = nullptr, *d_w = nullptr;
   int fd_K = 0, fd_M = 0;
   // tiling of the Chebyshev kernel (0 / -1 fields = auto)
-  ltb_kpe_tiling tiling = {0, 0, 0, 0, 0, -1};
+  ltb_kpe_tiling tiling = {0, 0, 0, 0, 0, -1, 0};
   // statistics of the last ltb_kpe_moments
   int stat_launches = 0;
   double stat_seconds = 0, stat_bytes = 0;
@@ -78,7 +79,9 @@ int set_err(ltb_ctx* c, int code, const std::string& msg);
       return set_err((ctx), LTB_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-// helpers implemented in ltb_capi.hip, used by sp2.hip
+// helpers implemented in ltb_capi.hip, used by sp2.hip and assemble.hip
+void release_system(ltb_ctx* c);
+int alloc_system(ltb_ctx* c, int nat, int nnzb, int gnnz);
 int ensure_buffer(ltb_ctx* c, void** p, size_t* cap, size_t bytes);
 int kpe_moments_impl(ltb_ctx* c, int M, int precision);
 int kpe_time_steps(ltb_ctx* c, int precision, int steps, double* seconds, double* bytes);

@@ -418,7 +418,7 @@ void free_all(ltb_ctx* c) {
   void* ptrs[] = {c->d_rowptr, c->d_col, c->d_blkrow, c->d_h0, c->d_ht64, c->d_ht32, c->d_offabs,
                   c->d_diag0, c->d_grow, c->d_gcol, c->d_gval, c->d_p0, c->d_pin, c->d_pout,
                   c->d_v, c->d_sc, c->d_red, c->d_xa, c->d_xb, c->d_part, c->d_mu1, c->d_mom,
-                  c->d_W, c->d_X, c->d_g, c->d_c, c->d_w, c->d_sa, c->d_sb, c->d_momorb};
+                  c->d_W, c->d_X, c->d_g, c->d_c, c->d_w, c->d_sa, c->d_sb, c->d_momorb, c->d_visit};
   for (void* p : ptrs)
     if (p) hipFree(p);
 }
@@ -571,20 +571,54 @@ void ltb_destroy(ltb_ctx* c) {
 
 const char* ltb_last_error(const ltb_ctx* c) { return c ? c->err.c_str() : g_sys_err.c_str(); }
 
+}  // extern "C"
+
+// Drop the buffers sized by the loaded system.
+void release_system(ltb_ctx* c) {
+  void** ps[] = {(void**)&c->d_rowptr, (void**)&c->d_col, (void**)&c->d_blkrow, (void**)&c->d_h0,
+                 (void**)&c->d_ht64, (void**)&c->d_ht32, (void**)&c->d_offabs, (void**)&c->d_diag0,
+                 (void**)&c->d_grow, (void**)&c->d_gcol, (void**)&c->d_gval, (void**)&c->d_p0,
+                 (void**)&c->d_pin, (void**)&c->d_pout, (void**)&c->d_v, (void**)&c->d_visit};
+  for (void** p : ps) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  c->mom_M = -1;
+  c->nat = c->norb = c->nnzb = c->gnnz = 0;
+  c->nelectrons = 0;
+}
+
+// Allocate the buffers of a system with nat atoms, nnzb H blocks and gnnz
+// Gamma entries (contents not set).
+int alloc_system(ltb_ctx* c, int nat, int nnzb, int gnnz) {
+  const int norb = 4 * nat;
+  LTB_HIP(c, hipMalloc(&c->d_rowptr, sizeof(int) * (nat + 1)));
+  LTB_HIP(c, hipMalloc(&c->d_col, sizeof(int) * nnzb));
+  LTB_HIP(c, hipMalloc(&c->d_blkrow, sizeof(int) * nnzb));
+  LTB_HIP(c, hipMalloc(&c->d_h0, sizeof(double) * 16 * nnzb));
+  LTB_HIP(c, hipMalloc(&c->d_ht64, sizeof(double) * 16 * nnzb));
+  LTB_HIP(c, hipMalloc(&c->d_ht32, sizeof(float) * 16 * nnzb));
+  LTB_HIP(c, hipMalloc(&c->d_offabs, sizeof(double) * norb));
+  LTB_HIP(c, hipMalloc(&c->d_diag0, sizeof(double) * norb));
+  LTB_HIP(c, hipMalloc(&c->d_grow, sizeof(int) * (nat + 1)));
+  LTB_HIP(c, hipMalloc(&c->d_gcol, sizeof(int) * gnnz));
+  LTB_HIP(c, hipMalloc(&c->d_gval, sizeof(double) * gnnz));
+  LTB_HIP(c, hipMalloc(&c->d_p0, sizeof(double) * nat));
+  LTB_HIP(c, hipMalloc(&c->d_pin, sizeof(double) * nat));
+  LTB_HIP(c, hipMalloc(&c->d_pout, sizeof(double) * nat));
+  LTB_HIP(c, hipMalloc(&c->d_v, sizeof(double) * nat));
+  LTB_HIP(c, hipMalloc(&c->d_visit, sizeof(int) * nat));
+  LTB_HIP(c, hipMemset(c->d_v, 0, sizeof(double) * nat));
+  return LTB_OK;
+}
+
+extern "C" {
+
 int ltb_load_system(ltb_ctx* c, const ltb_system* s) {
   if (!c || !s) return LTB_ERR_ARG;
   LTB_HIP(c, hipSetDevice(c->device));
   LTB_HIP(c, hipStreamSynchronize(c->stream));
-  // drop previous system-sized buffers
-  void** ps[] = {(void**)&c->d_rowptr, (void**)&c->d_col, (void**)&c->d_blkrow, (void**)&c->d_h0,
-                 (void**)&c->d_ht64, (void**)&c->d_ht32, (void**)&c->d_offabs, (void**)&c->d_diag0,
-                 (void**)&c->d_grow, (void**)&c->d_gcol, (void**)&c->d_gval, (void**)&c->d_p0,
-                 (void**)&c->d_pin, (void**)&c->d_pout, (void**)&c->d_v};
-  for (void** p : ps) {
-    if (*p) hipFree(*p);
-    *p = nullptr;
-  }
-  c->mom_M = -1;
+  release_system(c);
   const int nat = s->natoms, nnzb = (int)s->h_col.size(), gnnz = (int)s->g_col.size();
   const int norb = 4 * nat;
   std::vector<int> blkrow(nnzb);
@@ -601,21 +635,14 @@ int ltb_load_system(ltb_ctx* c, const ltb_system* s) {
             offabs[4 * i + a] += std::fabs(v);
         }
     }
-  LTB_HIP(c, hipMalloc(&c->d_rowptr, sizeof(int) * (nat + 1)));
-  LTB_HIP(c, hipMalloc(&c->d_col, sizeof(int) * nnzb));
-  LTB_HIP(c, hipMalloc(&c->d_blkrow, sizeof(int) * nnzb));
-  LTB_HIP(c, hipMalloc(&c->d_h0, sizeof(double) * 16 * nnzb));
-  LTB_HIP(c, hipMalloc(&c->d_ht64, sizeof(double) * 16 * nnzb));
-  LTB_HIP(c, hipMalloc(&c->d_ht32, sizeof(float) * 16 * nnzb));
-  LTB_HIP(c, hipMalloc(&c->d_offabs, sizeof(double) * norb));
-  LTB_HIP(c, hipMalloc(&c->d_diag0, sizeof(double) * norb));
-  LTB_HIP(c, hipMalloc(&c->d_grow, sizeof(int) * (nat + 1)));
-  LTB_HIP(c, hipMalloc(&c->d_gcol, sizeof(int) * gnnz));
-  LTB_HIP(c, hipMalloc(&c->d_gval, sizeof(double) * gnnz));
-  LTB_HIP(c, hipMalloc(&c->d_p0, sizeof(double) * nat));
-  LTB_HIP(c, hipMalloc(&c->d_pin, sizeof(double) * nat));
-  LTB_HIP(c, hipMalloc(&c->d_pout, sizeof(double) * nat));
-  LTB_HIP(c, hipMalloc(&c->d_v, sizeof(double) * nat));
+  std::vector<int> visit;
+  ltb::Status vs = ltb::locality_order(*s, visit);
+  if (vs.code) return set_err(c, LTB_ERR_ARG, vs.msg);
+  int st = alloc_system(c, nat, nnzb, gnnz);
+  if (st) {
+    release_system(c);
+    return st;
+  }
   LTB_HIP(c, hipMemcpy(c->d_rowptr, s->h_rowptr.data(), sizeof(int) * (nat + 1), hipMemcpyHostToDevice));
   LTB_HIP(c, hipMemcpy(c->d_col, s->h_col.data(), sizeof(int) * nnzb, hipMemcpyHostToDevice));
   LTB_HIP(c, hipMemcpy(c->d_blkrow, blkrow.data(), sizeof(int) * nnzb, hipMemcpyHostToDevice));
@@ -627,7 +654,7 @@ int ltb_load_system(ltb_ctx* c, const ltb_system* s) {
   LTB_HIP(c, hipMemcpy(c->d_gval, s->g_val.data(), sizeof(double) * gnnz, hipMemcpyHostToDevice));
   LTB_HIP(c, hipMemcpy(c->d_p0, s->p0.data(), sizeof(double) * nat, hipMemcpyHostToDevice));
   LTB_HIP(c, hipMemcpy(c->d_pin, s->p0.data(), sizeof(double) * nat, hipMemcpyHostToDevice));
-  LTB_HIP(c, hipMemset(c->d_v, 0, sizeof(double) * nat));
+  LTB_HIP(c, hipMemcpy(c->d_visit, visit.data(), sizeof(int) * nat, hipMemcpyHostToDevice));
   c->nat = nat;
   c->norb = norb;
   c->nnzb = nnzb;
@@ -650,6 +677,15 @@ int ltb_get_shift(ltb_ctx* c, double* v) {
   if (st) return st;
   if (!v) return set_err(c, LTB_ERR_ARG, "null output");
   LTB_HIP(c, hipMemcpyAsync(v, c->d_v, sizeof(double) * c->nat, hipMemcpyDeviceToHost, c->stream));
+  LTB_HIP(c, hipStreamSynchronize(c->stream));
+  return LTB_OK;
+}
+
+int ltb_get_mixed_populations(ltb_ctx* c, double* p) {
+  int st = need_system(c);
+  if (st) return st;
+  if (!p) return set_err(c, LTB_ERR_ARG, "null output");
+  LTB_HIP(c, hipMemcpyAsync(p, c->d_pin, sizeof(double) * c->nat, hipMemcpyDeviceToHost, c->stream));
   LTB_HIP(c, hipStreamSynchronize(c->stream));
   return LTB_OK;
 }
@@ -863,8 +899,8 @@ int ltb_set_kpe_timing(ltb_ctx* c, int on) {
 int ltb_set_kpe_tiling(ltb_ctx* c, const ltb_kpe_tiling* t) {
   if (!c || !t) return LTB_ERR_ARG;
   if (t->chunk_cols < 0 || t->atoms_per_wave < 0 || t->blocks_per_cu < 0 || t->schedule < 0 ||
-      t->schedule > 2 || (t->cols_per_lane != 0 && t->cols_per_lane != 1 && t->cols_per_lane != 2 &&
-                          t->cols_per_lane != 4))
+      t->schedule > 2 || t->order < 0 || t->order > 2 ||
+      (t->cols_per_lane != 0 && t->cols_per_lane != 1 && t->cols_per_lane != 2 && t->cols_per_lane != 4))
     return set_err(c, LTB_ERR_ARG, "bad tiling");
   c->tiling = *t;
   return LTB_OK;
@@ -889,8 +925,8 @@ int ltb_kpe_autotune(ltb_ctx* c, int precision, int trial_steps, ltb_kpe_tiling*
   for (int ch : chunks)
     for (int v : vs)
       for (int nt = 0; nt <= 1; nt++) {
-        cand.push_back({ch, v, LTB_SCHED_BLOCKED, 0, 0, nt});
-        for (int bpc : {2, 4, 8}) cand.push_back({ch, v, LTB_SCHED_SWEEP, bpc, 0, nt});
+        cand.push_back({ch, v, LTB_SCHED_BLOCKED, 0, 0, nt, saved.order});
+        for (int bpc : {2, 4, 8}) cand.push_back({ch, v, LTB_SCHED_SWEEP, bpc, 0, nt, saved.order});
       }
   double best_rate = -1;
   ltb_kpe_tiling win = saved;

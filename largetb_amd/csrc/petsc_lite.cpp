@@ -15,10 +15,21 @@ int fail(State& s, int code) {
   return code;
 }
 
+int io_fail(State& s, int code) {
+  if (code != LTB_OK) s.last_error = ltb_io_last_error();
+  return code;
+}
+
 #define CHK(s, call)                  \
   do {                                \
     int e_ = (call);                  \
     if (e_ != LTB_OK) return fail(s, e_); \
+  } while (0)
+
+#define CHKIO(s, call)                   \
+  do {                                   \
+    int e_ = (call);                     \
+    if (e_ != LTB_OK) return io_fail(s, e_); \
   } while (0)
 
 // apply getSP2 at depth `depth` to the unit vectors e_j0..e_j1-1 and return
@@ -32,6 +43,8 @@ int diag_block(State& s, int depth, int j0, int j1, std::vector<double>& out) {
   }
   return LTB_OK;
 }
+
+std::string dump_path(const State& s, const char* file) { return s.opt.dump_dir + "/" + file; }
 
 }  // namespace
 
@@ -71,6 +84,9 @@ PetscErrorCode constructH(State& s) {
   s.H.col.resize(nnzb);
   s.H.val.resize(16 * (size_t)nnzb);
   CHK(s, ltb_system_hamiltonian(s.sys, s.H.rowptr.data(), s.H.col.data(), s.H.val.data()));
+  if (!s.opt.dump_dir.empty())  // MatView(H) in MATLAB format (:1504-1509)
+    CHKIO(s, ltb_write_matlab_bsr(dump_path(s, "H.m").c_str(), "H", s.H.nrows, s.H.nrows, 4, s.H.rowptr.data(),
+                                s.H.col.data(), s.H.val.data()));
   return LTB_OK;
 }
 
@@ -85,6 +101,9 @@ PetscErrorCode initializeForSCF(State& s) {
   s.Gamma_ab.val.resize(gnnz);
   CHK(s, ltb_system_gamma(s.sys, s.Gamma_ab.rowptr.data(), s.Gamma_ab.col.data(), s.Gamma_ab.val.data()));
   s.diagP.assign(s.nSize, 0.0);  // initializeDiagP (:1725-1740)
+  if (!s.opt.dump_dir.empty())  // MatView(Gamma_ab) (:950-955)
+    CHKIO(s, ltb_write_matlab_bsr(dump_path(s, "GAMMA_AB.m").c_str(), "GAMMA_AB", s.Natoms, s.Natoms, 1,
+                                s.Gamma_ab.rowptr.data(), s.Gamma_ab.col.data(), s.Gamma_ab.val.data()));
   return LTB_OK;
 }
 
@@ -191,6 +210,22 @@ PetscErrorCode constructdeltaH(State& s) {
   CHK(s, ltb_set_populations(s.ctx, s.mixedCharges.data()));
   s.gammaq.assign(s.Natoms, 0.0);
   CHK(s, ltb_get_shift(s.ctx, s.gammaq.data()));
+  if (!s.opt.dump_dir.empty()) {
+    // deltaH as 4x4 diagonal blocks, netCharges and gammaq (:1465-1484)
+    std::vector<int> rp(s.Natoms + 1), col(s.Natoms);
+    std::vector<double> blk(16 * (size_t)s.Natoms, 0.0);
+    for (int i = 0; i < s.Natoms; i++) {
+      rp[i + 1] = i + 1;
+      col[i] = i;
+      for (int a = 0; a < 4; a++) blk[16 * (size_t)i + 5 * a] = s.gammaq[i];
+    }
+    CHKIO(s, ltb_write_matlab_bsr(dump_path(s, "deltaH.m").c_str(), "deltaH", s.Natoms, s.Natoms, 4, rp.data(),
+                                col.data(), blk.data()));
+    CHKIO(s, ltb_write_matlab_vec(dump_path(s, "netCharge.m").c_str(), "Vec_netCharge", s.Natoms,
+                                s.netCharges.data()));
+    CHKIO(s, ltb_write_matlab_vec(dump_path(s, "GammaQ_vector.m").c_str(), "Vec_GammaQ", s.Natoms,
+                                s.gammaq.data()));
+  }
   return LTB_OK;
 }
 

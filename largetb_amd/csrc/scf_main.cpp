@@ -2,7 +2,7 @@
 // main :1742-1849) over the entry points of ltb_petsc.hpp.  The trace is exact
 // here, so the 4th argument of the reference (nRandoms) is not taken.
 //   ltb_scf L M N [--sigma A] [--seed S] [--moments M] [--kT eV] [--reference]
-//                 [--maxsp2 D] [--tol Q] [--max-scf K] [--fp32] [--sp2-alpha a]
+//                 [--maxsp2 D] [--tol Q] [--max-scf K] [--fp32] [--sp2-alpha a] [--dump DIR]
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -20,7 +20,7 @@ static double now() {
 int main(int argc, char** argv) {
   if (argc < 4) {
     std::fprintf(stderr, "usage: %s L M N [--sigma A] [--seed S] [--moments M] [--kT eV] "
-                 "[--reference] [--maxsp2 D] [--tol Q] [--max-scf K] [--fp32]\n", argv[0]);
+                 "[--reference] [--maxsp2 D] [--tol Q] [--max-scf K] [--fp32] [--sp2-alpha a] [--dump DIR]\n", argv[0]);
     return 2;
   }
   State s;
@@ -40,6 +40,7 @@ int main(int argc, char** argv) {
     else if (a == "--max-scf") s.opt.max_scf = std::atoi(val());
     else if (a == "--fp32") s.opt.precision = LTB_FP32_ACC64;
     else if (a == "--sp2-alpha") s.opt.force_alpha = std::atof(val());
+    else if (a == "--dump") s.opt.dump_dir = val();
     else {
       std::fprintf(stderr, "unknown option %s\n", a.c_str());
       return 2;

@@ -43,4 +43,8 @@ Status sk_block(int zi, int zj, double x, double y, double z, double h[16]);
 // [rowptr[i], rowptr[i+1]) hold j and the displacement r_j + image - r_i.
 Status pairs_within(const ltb_system& s, double rc, std::vector<int>& rowptr,
                     std::vector<int>& nbj, std::vector<double>& nbd);
+
+// Visiting order of the atoms that keeps bonded atoms close in sequence
+// (strips of spatial bins); a permutation of 0..natoms-1.
+Status locality_order(const ltb_system& s, std::vector<int>& order);
 }  // namespace ltb

@@ -2,9 +2,12 @@
 
 The training structures follow the batched workload: alpha-quartz supercells
 with L, M in {2..5}, N in {2..4} and a displacement sigma uniform in
-[0.02, 0.15] A, drawn from a SplitMix64 stream.  Each is solved by the plain
-SCC (FP64, neutral start); its descriptors and converged charges form the
-data.  Training runs the device kernels (ltb_mlp_adam).
+[0.02, 0.15] A, drawn from a SplitMix64 stream.  Each is solved from the
+neutral start without any guess: either one at a time by the FP64 SCC
+(`generate`) or 256 at a time by the batched SCC with an FP32 bulk and FP64
+finish (`generate_batched`, charges within ~1e-8 e of the FP64 solve).  The
+descriptors and converged charges form the data.  Training runs the device
+kernels (ltb_mlp_adam).
 """
 import json
 import os
@@ -12,7 +15,7 @@ import time
 
 import numpy as np
 
-from . import FP64, Context, Mlp, System
+from . import FP32_ACC64, FP64, Batch, Context, Mlp, System
 
 MASK = (1 << 64) - 1
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "guess_weights.npz")
@@ -67,6 +70,49 @@ def generate(ctx, structures, M=512, kT_ev=0.05, log=None):
                 seg=np.array(seg, np.int32), qtot=np.array(qtot), meta=meta)
 
 
+def solve_batched(ctx, structures, M=256, kT_ev=0.05, per_batch=256, log=None):
+    """Converged charges of every structure, `per_batch` structures per batched
+    SCC (FP32 bulk, FP64 finish, |dq| < 1e-6 e).  Returns the per-atom charges
+    (e, structure order) and per-structure atom offsets, iterations, status."""
+    qs, offs, iters, status = [], [0], [], []
+    t0 = time.perf_counter()
+    for b0 in range(0, len(structures), per_batch):
+        systems = [System.quartz(L, Mc, N, sigma_ang=sg, seed=sd)
+                   for L, Mc, N, sg, sd in structures[b0:b0 + per_batch]]
+        with Batch(ctx, systems) as b:
+            if b0 == 0:
+                b.autotune(FP32_ACC64, 3)
+            r = b.scc(M, kT_ev=kT_ev, precision=FP32_ACC64, switch_tol=1e-6, tol=1e-6, max_iter=400,
+                      use_graph=1, check=False)
+        qs.append(r["q"])
+        for s in systems:
+            offs.append(offs[-1] + s.nat)
+        iters.extend(r["iters"].tolist())
+        status.extend(r["struct_status"].tolist())
+        if log:
+            log(f"batched solve: {min(b0 + per_batch, len(structures))}/{len(structures)} structures, "
+                f"{offs[-1]} atoms, {time.perf_counter() - t0:.1f} s")
+    return (np.concatenate(qs), np.array(offs, np.int64), np.array(iters, np.int32),
+            np.array(status, np.int32))
+
+
+def assemble(ctx, structures, q, offs, status):
+    """Descriptors and targets of the converged structures (status 0)."""
+    feats, Zs, qk, seg, qtot = [], [], [], [0], []
+    for k, (L, Mc, N, sg, sd) in enumerate(structures):
+        if status[k] != 0:
+            continue
+        s = System.quartz(L, Mc, N, sigma_ang=sg, seed=sd)
+        Z, _ = s.coords()
+        feats.append(ctx.descriptors(s))
+        Zs.append(Z)
+        qk.append(q[offs[k]:offs[k + 1]])
+        seg.append(seg[-1] + s.nat)
+        qtot.append(s.nelectrons)
+    return dict(feat=np.concatenate(feats), Z=np.concatenate(Zs), q=np.concatenate(qk),
+                seg=np.array(seg, np.int32), qtot=np.array(qtot))
+
+
 def train(ctx, data, steps=3000, lr=2e-3, seed=1, log=None):
     m = Mlp(ctx, seed=seed)
     m.set_data(data["feat"], data["Z"], data["seg"], data["qtot"], data["q"])
@@ -74,7 +120,7 @@ def train(ctx, data, steps=3000, lr=2e-3, seed=1, log=None):
     hist = []
     done = 0
     while done < steps:
-        k = min(500, steps - done)
+        k = min(100, steps - done)
         h = m.adam(k, lr=lr if done < steps // 2 else lr / 4)
         hist.extend(h.tolist())
         done += k
@@ -96,14 +142,24 @@ def load(ctx, path=WEIGHTS):
 
 def main(argv=None):
     """python -m largetb_amd.learned [--count 256] [--seed 2026] [--M 512] [--steps 3000]
-    generates the training set on the GPU, trains, and writes guess_weights.npz"""
+    generates the training set on the GPU, trains, and writes guess_weights.npz.
+    --batched solves 256 structures per batched SCC; --charges-out writes only
+    the converged charges (with --skip-train, a generation-only run);
+    --charges-in trains from charges files written that way (several files =
+    consecutive slices of the same structure stream)."""
     import argparse
     import sys
     ap = argparse.ArgumentParser()
     ap.add_argument("--count", type=int, default=256)
+    ap.add_argument("--first", type=int, default=0, help="skip this many structures of the stream")
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--M", type=int, default=512)
     ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--batched", action="store_true")
+    ap.add_argument("--charges-out", default="")
+    ap.add_argument("--charges-in", nargs="*", default=[])
+    ap.add_argument("--skip-train", action="store_true")
     ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--data-out", default="")
     a = ap.parse_args(argv)
@@ -113,20 +169,51 @@ def main(argv=None):
 
     ctx = Context(0)
     t0 = time.perf_counter()
-    data = generate(ctx, batch_structures(a.count, a.seed), M=a.M, log=log)
+    if a.charges_in:
+        parts = [np.load(p, allow_pickle=False) for p in a.charges_in]
+        first = int(parts[0]["first"])
+        count = sum(int(p["count"]) for p in parts)
+        structures = batch_structures(first + count, a.seed)[first:]
+        q = np.concatenate([p["q"].astype(np.float64) for p in parts])
+        offs = [0]
+        for p in parts:
+            offs.extend((offs[-1] + p["offs"][1:]).tolist())
+        status = np.concatenate([p["status"] for p in parts])
+        iters = np.concatenate([p["iters"] for p in parts])
+        data = assemble(ctx, structures, q, np.array(offs), status)
+        M = int(parts[0]["M"])
+    elif a.batched:
+        structures = batch_structures(a.first + a.count, a.seed)[a.first:]
+        q, offs, iters, status = solve_batched(ctx, structures, M=a.M, log=log)
+        if a.charges_out:
+            np.savez_compressed(a.charges_out, q=q.astype(np.float32), offs=offs, iters=iters,
+                                status=status, first=a.first, count=a.count, seed=a.seed, M=a.M)
+        if a.skip_train:
+            print(json.dumps(dict(count=a.count, first=a.first, atoms=int(offs[-1]),
+                                  converged=int((status == 0).sum()), mean_iters=float(iters.mean()),
+                                  seconds=round(time.perf_counter() - t0, 1))))
+            ctx.close()
+            return
+        data = assemble(ctx, structures, q, offs, status)
+        M = a.M
+    else:
+        data = generate(ctx, batch_structures(a.count, a.seed), M=a.M, log=log)
+        iters = np.array([x["iters"] for x in data["meta"]])
+        M = a.M
     tgen = time.perf_counter() - t0
+    log(f"data: {len(data['seg']) - 1} structures, {len(data['Z'])} atoms, {tgen:.1f} s")
     if a.data_out:
         np.savez_compressed(a.data_out, feat=data["feat"], Z=data["Z"], q=data["q"], seg=data["seg"],
                             qtot=data["qtot"])
     t0 = time.perf_counter()
-    m, hist = train(ctx, data, steps=a.steps, log=log)
+    m, hist = train(ctx, data, steps=a.steps, lr=a.lr, log=log)
     ttrain = time.perf_counter() - t0
     q = m.forward()
     err = np.abs(q - data["q"])
-    info = dict(count=a.count, seed=a.seed, M=a.M, atoms=int(len(data["Z"])), steps=a.steps,
+    info = dict(count=len(data["seg"]) - 1, seed=a.seed, M=M, atoms=int(len(data["Z"])), steps=a.steps,
                 train_mse=float(hist[-1]), train_max_err=float(err.max()),
                 generate_s=round(tgen, 1), train_s=round(ttrain, 1),
-                mean_scc_iters=float(np.mean([x["iters"] for x in data["meta"]])))
+                mean_scc_iters=float(np.mean(iters)))
     save(m, a.out, info)
     print(json.dumps(info))
     m.close()

@@ -86,18 +86,19 @@ def test_ragged_chunks_equal_single_chunk(ctx, prec):
     assert np.abs(got2 - ref).max() < tol
 
 
-TILINGS = [dict(schedule=sch, cols_per_lane=v, nontemporal=nt, chunk_cols=ch, blocks_per_cu=bpc)
-           for sch, v, nt, ch, bpc in [
-               (L.SCHED_BLOCKED, 1, 0, 4096, 0), (L.SCHED_BLOCKED, 2, 1, 2048, 0),
-               (L.SCHED_BLOCKED, 4, 1, 4096, 0), (L.SCHED_SWEEP, 1, 1, 4096, 4),
-               (L.SCHED_SWEEP, 2, 0, 4096, 2), (L.SCHED_SWEEP, 4, 1, 8192, 8),
-               (L.SCHED_SWEEP, 1, 1, 5000, 3)]]
+TILINGS = [dict(schedule=sch, cols_per_lane=v, nontemporal=nt, chunk_cols=ch, blocks_per_cu=bpc, order=od)
+           for sch, v, nt, ch, bpc, od in [
+               (L.SCHED_BLOCKED, 1, 0, 4096, 0, 0), (L.SCHED_BLOCKED, 2, 1, 2048, 0, 0),
+               (L.SCHED_BLOCKED, 4, 1, 4096, 0, 0), (L.SCHED_SWEEP, 1, 1, 4096, 4, 1),
+               (L.SCHED_SWEEP, 2, 0, 4096, 2, 2), (L.SCHED_SWEEP, 4, 1, 8192, 8, 1),
+               (L.SCHED_SWEEP, 4, 1, 4096, 2, 2), (L.SCHED_SWEEP, 1, 1, 5000, 3, 1)]]
 
 
 @pytest.mark.parametrize("prec", [L.FP64, L.FP32_ACC64])
 def test_every_tiling_gives_the_same_moments(ctx, prec):
     """18,432 orbitals: both schedules, V in {1,2,4}, cached / non-temporal,
-    chunk widths that leave a ragged last chunk (5000 -> 3 x 5056 + 3264)."""
+    sweep in spatial-strip and in atom-index order, chunk widths that leave a
+    ragged last chunk (5000 -> 3 x 5056 + 3264)."""
     s = L.System.quartz(8, 8, 8, sigma_ang=0.05, seed=1)
     prepare(ctx, s)
     try:
@@ -219,6 +220,20 @@ def test_scc_graph_replays_match_eager(ctx, c0):
         assert graph["iters"] == eager["iters"]
         assert np.array_equal(graph["q"], eager["q"])
         assert np.array_equal(graph["hist"], eager["hist"])
+
+
+def test_scc_resumes_exactly_from_mixed_populations(ctx, c0):
+    """An SCC cut after 10 iterations and restarted from the mixed populations
+    repeats the uninterrupted run bit for bit (linear mixing has no other state)."""
+    s, _ = c0
+    ctx.load(s)
+    full = ctx.scc(200, precision=L.FP64, tol=1e-6, max_iter=300)
+    first = ctx.scc(200, precision=L.FP64, tol=1e-6, max_iter=10, check=False)
+    assert first["status"] == L.LTB_NOT_CONVERGED
+    rest = ctx.scc(200, guess=ctx.mixed_populations(), precision=L.FP64, tol=1e-6, max_iter=300)
+    assert rest["iters"] == full["iters"] - 10
+    assert np.array_equal(rest["q"], full["q"])
+    assert np.array_equal(np.concatenate([first["hist"], rest["hist"]]), full["hist"])
 
 
 def test_scc_iteration_cap_reports_not_converged(ctx, c0):

@@ -73,3 +73,33 @@ def test_sp2_rule_stops_when_trace_is_exact(tmp_path):
     o, V, f = p0_eigen()
     pop = ((V ** 2) @ f).reshape(-1, 4).sum(1)
     assert np.abs(q - 2 * pop).max() < 1e-8
+
+
+@pytest.mark.gpu
+def test_driver_writes_the_reference_dumps(tmp_path, ref_dumps):
+    """--dump writes the reference's MATLAB files at the reference's points:
+    H.m and GAMMA_AB.m of the reference-lattice 4x4x4 cell equal the
+    reference's own dumps entry for entry; deltaH.m / netCharge.m /
+    GammaQ_vector.m of the last SCC iteration satisfy GammaQ = Gamma . netCharge
+    and deltaH = diag(repeat(GammaQ, 4))."""
+    import largetb_amd as L
+    d = tmp_path / "dump"
+    d.mkdir()
+    r, out, q = run(["4", "4", "4", "--reference", "--max-scf", "2", "--dump", str(d)], tmp_path)
+    assert r.returncode in (0, 5), r.stderr
+    shape, rr, cc, vv = L.read_matlab_mat(d / "H.m")
+    assert np.array_equal(rr, ref_dumps["H_row"]) and np.array_equal(cc, ref_dumps["H_col"])
+    assert np.abs(vv - ref_dumps["H_val"]).max() < 2e-15
+    shape, rr, cc, vv = L.read_matlab_mat(d / "GAMMA_AB.m")
+    assert np.array_equal(rr, ref_dumps["GAMMA_AB_row"]) and np.array_equal(cc, ref_dumps["GAMMA_AB_col"])
+    G = np.zeros(shape)
+    G[rr, cc] = vv
+    dq = L.read_matlab_vec(d / "netCharge.m")
+    gq = L.read_matlab_vec(d / "GammaQ_vector.m")
+    assert np.abs(dq).max() > 1e-3  # second iteration: charges have moved
+    assert np.abs(G @ dq - gq).max() < 1e-12
+    shape, rr, cc, vv = L.read_matlab_mat(d / "deltaH.m")
+    assert shape == (4 * len(gq), 4 * len(gq)) and len(vv) == 16 * len(gq)
+    D = np.zeros(shape)
+    D[rr, cc] = vv
+    assert np.array_equal(D, np.diag(np.repeat(gq, 4)))
