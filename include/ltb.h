@@ -122,6 +122,10 @@ int ltb_get_gamma(ltb_ctx* ctx, int* rowptr, int* col, double* val);
 /* constructdeltaH (:1414-1487): v = Gamma (p - p0) on device (FP64), H_total =
  * H0 + diag(v repeated over the 4 orbitals).  p: per-spin populations [natoms]. */
 int ltb_set_populations(ltb_ctx* ctx, const double* p);
+/* replace the neutral per-spin populations p0 of Delta q = p - p0 (default
+ * Si 2, O 3); the reference's literal values (Si 2, O 0: :714-720) for
+ * comparisons with its runs */
+int ltb_set_neutral(ltb_ctx* ctx, const double* p0);
 int ltb_get_shift(ltb_ctx* ctx, double* v_atom);
 
 /* GershGorinParallel (:144-209) on H_total, FP64 device reduction. */
@@ -169,6 +173,11 @@ int ltb_set_kpe_orbital_moments(ltb_ctx* ctx, int on);
  * as getKPE applies them (:1134-1166) -- input of ltb_kpe_apply */
 int ltb_kpe_weights(ltb_ctx* ctx, int M, double* w);
 int ltb_density_diag(ltb_ctx* ctx, double* diagP);
+/* the weights w[m] (as ltb_kpe_weights) at a given chemical potential mu_ha
+ * (Hartree; e.g. runKPM's mu from the DOS, :1681-1712) in the frame of the
+ * last ltb_rescale: Fermi-Dirac at kT (Hartree) by K-node quadrature, or the
+ * reference's step function (:1100-1103) for kT <= 0 */
+int ltb_kpe_weights_at(ltb_ctx* ctx, int M, double mu_ha, double kT, int K, double* w);
 
 typedef struct {
   int M;              /* Chebyshev moments                                       */
@@ -242,6 +251,34 @@ int ltb_batch_scc_run(ltb_batch* b, const ltb_scc_params* prm, const double* p_g
  * read by the gathers, read as t_{n-1} and written as t_{n+1} (1, 2, 3
  * streams for steps 0, 1, >1) plus its H~ blocks, columns and row pointers. */
 int ltb_batch_stats(const ltb_batch* b, long long* launches, double* seconds, double* bytes);
+
+/* ---- stochastic-trace mode (the reference's default estimator) ----------------
+ * Moments r^T T_m(H~) r, m = 0..M, of given vectors R ([nvec][norb], host) by
+ * the exact-trace block recursion with explicit start vectors (moment
+ * doubling): moments [nvec][M+1] and trace[m] = sum over the vectors (either
+ * may be NULL).  With R = unit vectors this is the exact trace; with Gaussian
+ * R it is runKPM's momentsTrace (:1607-1633).  H~ from ltb_rescale. */
+int ltb_kpe_block_moments(ltb_ctx* ctx, int M, int nvec, const double* R, int precision, double* moments,
+                          double* trace);
+/* nvec x n standard normal numbers (vector-major) from the SplitMix64 ->
+ * Box-Muller stream of seed -- the vectors the two calls below use */
+int ltb_random_vectors(unsigned long long seed, int nvec, int n, double* R);
+/* runKPM's stochastic moments (:1607-1633) with nvec seeded Gaussian vectors */
+int ltb_kpe_stochastic(ltb_ctx* ctx, int M, int nvec, unsigned long long seed, int precision, double* trace);
+/* computeDiagonalElements, random mode (:1338-1392): diagP = sum_r r.*(P r) ./
+ * sum_r r.*r, P r = sum_m w[m] T_m(H~) r (getKPE on the block, w as for
+ * ltb_kpe_apply), the same seeded vectors */
+int ltb_density_stochastic(ltb_ctx* ctx, int M, const double* w, int nvec, unsigned long long seed,
+                           int precision, double* diagP);
+/* runKPM's DOS and mu (:1634-1712), host arithmetic on nmom trace moments:
+ * Jackson kernel g_n (N = nmom), DOS at x_k = cos(pi (k+1/2)/ntilde), energies
+ * a x_k + b; mu where the DOS area from the bottom reaches Ne/(2 nsize) of the
+ * total; emin / emax where DOS/nsize first reaches 0.001 from either end (the
+ * reference's bound tightening).  Any output may be NULL. */
+int ltb_kpm_dos(const double* moments, int nmom, int ntilde, double a, double b, int nsize, double nelectrons,
+                double* energies, double* dos, double* mu, double* emin, double* emax);
+/* dosfile.txt rows "E, DOS" (:1714-1717) */
+int ltb_write_dos(const char* path, int n, const double* energies, const double* dos);
 
 /* getKPE (:1068-1185) on a block of vectors: Y = sum_m w[m] T_m(H~) X, with
  * w = (c_0/2, c_1 g_1, ...).  X, Y: [ncols][norb] column vectors (host). */

@@ -42,6 +42,14 @@ struct Settings {
   int max_scf = 20;                     // :1820
   int device = 0;
   bool verbose = true;
+  int nrandoms = 0;                     // nRandoms (argv[4], :1757): 0 = exact trace over unit vectors;
+                                        // > 0 = runKPM's stochastic DOS and mu (:1562-1712) and the
+                                        // random-vector diagonal (:1338-1392) with this many vectors
+  int dos_moments = 1000;               // Nmoments of the stochastic DOS (:1565)
+  bool reference_literal = false;       // the reference's own SCC, for comparison with its runs: neutral
+                                        // O charge 0 (:714-720 has no O branch), the loop measuring
+                                        // successive outputs |q_k - q_(k-1)| (:1830-1838); mix then plays
+                                        // mixRatio (Hamiltonian mixing equals charge mixing, Gamma linear)
   std::string dump_dir;                 // non-empty: write the reference's MATLAB dumps there (H.m,
                                         // GAMMA_AB.m, deltaH.m, netCharge.m, GammaQ_vector.m)
 };
@@ -54,7 +62,10 @@ struct State {
   double Nelectrons = 0;
   double muval = 0;                     // scaled chemical potential (:43)
   double a = 1, b = 0;                  // H~ = (H - b)/a
-  double eband = 0;                     // 2 Tr(P H), Hartree
+  double eband = 0;                     // 2 Tr(P H), Hartree (exact-trace mode)
+  double mu_ha = 0;                     // chemical potential, Hartree
+  unsigned long long draws = 0;         // random-vector sets drawn so far (each gets its own seed)
+  Vec dosEnergies, dosValues;           // runKPM's DOS (stochastic mode)
   Mat H, Gamma_ab;
   Vec neutralCharges, chargeOnAtom, previousCharges, netCharges, gammaq, diagP;
   Vec mixedCharges;                     // populations H_total is built from

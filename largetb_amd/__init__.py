@@ -109,6 +109,7 @@ SIGNATURES = {
     "ltb_get_hamiltonian": (C.c_int, [VP, PI, PI, PD]),
     "ltb_get_gamma": (C.c_int, [VP, PI, PI, PD]),
     "ltb_set_populations": (C.c_int, [VP, PD]),
+    "ltb_set_neutral": (C.c_int, [VP, PD]),
     "ltb_get_shift": (C.c_int, [VP, PD]),
     "ltb_get_mixed_populations": (C.c_int, [VP, PD]),
     "ltb_gershgorin": (C.c_int, [VP, PD, PD]),
@@ -117,11 +118,19 @@ SIGNATURES = {
     "ltb_density": (C.c_int, [VP, C.POINTER(FermiParams), PD, C.POINTER(DensityResult)]),
     "ltb_scc_run": (C.c_int, [VP, C.POINTER(SccParams), PD, PD, PD, C.POINTER(SccResult)]),
     "ltb_kpe_apply": (C.c_int, [VP, C.c_int, PD, C.c_int, PD, PD, C.c_int]),
+    "ltb_kpe_block_moments": (C.c_int, [VP, C.c_int, C.c_int, PD, C.c_int, PD, PD]),
+    "ltb_random_vectors": (C.c_int, [C.c_ulonglong, C.c_int, C.c_int, PD]),
+    "ltb_kpe_stochastic": (C.c_int, [VP, C.c_int, C.c_int, C.c_ulonglong, C.c_int, PD]),
+    "ltb_density_stochastic": (C.c_int, [VP, C.c_int, PD, C.c_int, C.c_ulonglong, C.c_int, PD]),
+    "ltb_kpm_dos": (C.c_int, [PD, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, PD, PD, PD, PD,
+                              PD]),
+    "ltb_write_dos": (C.c_int, [C.c_char_p, C.c_int, PD, PD]),
     "ltb_sp2_dense": (C.c_int, [VP, C.c_double, C.c_int, C.c_int, PD, PD, C.POINTER(Sp2Result)]),
     "ltb_kpe_stats": (C.c_int, [VP, PI, PD, PD]),
     "ltb_set_kpe_orbital_moments": (C.c_int, [VP, C.c_int]),
     "ltb_density_diag": (C.c_int, [VP, PD]),
     "ltb_kpe_weights": (C.c_int, [VP, C.c_int, PD]),
+    "ltb_kpe_weights_at": (C.c_int, [VP, C.c_int, C.c_double, C.c_double, C.c_int, PD]),
     "ltb_set_kpe_timing": (C.c_int, [VP, C.c_int]),
     "ltb_set_kpe_tiling": (C.c_int, [VP, C.POINTER(KpeTiling)]),
     "ltb_get_kpe_tiling": (C.c_int, [VP, C.c_int, C.POINTER(KpeTiling)]),
@@ -286,6 +295,36 @@ def read_matlab_mat(path):
     return (nr.value, nc.value), r, c, v
 
 
+def random_vectors(seed, nvec, n):
+    """The seeded Gaussian vectors of the stochastic mode, [nvec, n]."""
+    R = np.zeros((nvec, n))
+    st = lib().ltb_random_vectors(seed, nvec, n, _d(R))
+    if st:
+        raise LtbError(st, "bad arguments")
+    return R
+
+
+def kpm_dos(moments, a, b, nsize, nelectrons, ntilde=None):
+    """runKPM's DOS, mu and tightened bounds from trace moments (host)."""
+    mom = np.ascontiguousarray(moments, np.float64)
+    nt = 2 * len(mom) if ntilde is None else ntilde
+    E, D = np.zeros(nt), np.zeros(nt)
+    mu, lo, hi = C.c_double(), C.c_double(), C.c_double()
+    st = lib().ltb_kpm_dos(_d(mom), len(mom), nt, a, b, nsize, nelectrons, _d(E), _d(D), C.byref(mu), C.byref(lo),
+                           C.byref(hi))
+    if st:
+        raise LtbError(st, "bad arguments")
+    return dict(energies=E, dos=D, mu=mu.value, emin=lo.value, emax=hi.value)
+
+
+def write_dos(path, energies, dos):
+    E = np.ascontiguousarray(energies, np.float64)
+    D = np.ascontiguousarray(dos, np.float64)
+    st = lib().ltb_write_dos(str(path).encode(), len(E), _d(E), _d(D))
+    if st:
+        raise LtbError(st, f"cannot write {path}")
+
+
 def read_matlab_vec(path):
     n = C.c_longlong(0)
     _io_check(lib().ltb_read_matlab_vec(str(path).encode(), C.byref(n), None))
@@ -329,6 +368,8 @@ class Context:
         self.system = system
         self.nat = system.nat
         self.norb = system.norb
+        self.nnzb = system.nnzb
+        self.nelectrons = system.nelectrons
         return self
 
     def load_coords(self, Z, xyz_bohr, cell9=None, periodic_mask=7):
@@ -371,6 +412,9 @@ class Context:
         self._check(lib().ltb_get_shift(self._h, _d(v)))
         return v
 
+    def set_neutral(self, p0):
+        self._check(lib().ltb_set_neutral(self._h, _d(np.ascontiguousarray(p0, np.float64))))
+
     def mixed_populations(self):
         """Per-spin populations the next SCC iteration starts from."""
         p = np.zeros(self.nat)
@@ -394,7 +438,7 @@ class Context:
 
     def density(self, M, kT_ev=0.05, K=8192, Ne=None):
         kT = kT_ev / HARTREE_EV if kT_ev > 0 else 0.0
-        fp = FermiParams(M, kT, K, self.system.nelectrons if Ne is None else Ne)
+        fp = FermiParams(M, kT, K, self.nelectrons if Ne is None else Ne)
         pop = np.zeros(self.nat)
         r = DensityResult()
         self._check(lib().ltb_density(self._h, C.byref(fp), _d(pop), C.byref(r)))
@@ -415,6 +459,31 @@ class Context:
                     dq=r.dq, mu_ha=r.mu_ha, eband=r.eband, eband_fp=r.eband_fp,
                     graph_launches=r.graph_launches, seconds=r.seconds,
                     hist=hist[:r.iterations])
+
+    def kpe_weights_at(self, M, mu_ha, kT_ev=0.05, K=8192):
+        kT = kT_ev / HARTREE_EV if kT_ev > 0 else 0.0
+        w = np.zeros(M)
+        self._check(lib().ltb_kpe_weights_at(self._h, M, mu_ha, kT, K, _d(w)))
+        return w
+
+    def block_moments(self, M, R, precision=FP64):
+        """r^T T_m(H~) r for the rows r of R: ([nvec, M+1] moments, [M+1] trace)."""
+        R = np.ascontiguousarray(R, np.float64)
+        mom = np.zeros((R.shape[0], M + 1))
+        tr = np.zeros(M + 1)
+        self._check(lib().ltb_kpe_block_moments(self._h, M, R.shape[0], _d(R), precision, _d(mom), _d(tr)))
+        return mom, tr
+
+    def stochastic_moments(self, M, nvec, seed=1, precision=FP64):
+        tr = np.zeros(M + 1)
+        self._check(lib().ltb_kpe_stochastic(self._h, M, nvec, seed, precision, _d(tr)))
+        return tr
+
+    def density_stochastic(self, M, w, nvec, seed=1, precision=FP64):
+        w = np.ascontiguousarray(w, np.float64)
+        d = np.zeros(self.norb)
+        self._check(lib().ltb_density_stochastic(self._h, M, _d(w), nvec, seed, precision, _d(d)))
+        return d
 
     def keep_orbital_moments(self, on=True):
         self._check(lib().ltb_set_kpe_orbital_moments(self._h, 1 if on else 0))
@@ -442,7 +511,7 @@ class Context:
         diag = np.zeros(self.norb)
         pop = np.zeros(self.nat)
         r = Sp2Result()
-        self._check(lib().ltb_sp2_dense(self._h, self.system.nelectrons if Ne is None else Ne,
+        self._check(lib().ltb_sp2_dense(self._h, self.nelectrons if Ne is None else Ne,
                                         max_iter, fp64_tail, _d(diag), _d(pop), C.byref(r)))
         return dict(diag=diag, pop=pop, iters=r.iterations, fp64_iters=r.fp64_iterations,
                     trace=r.trace, idem=r.idem, gemm_seconds=r.gemm_seconds, gemm_flops=r.gemm_flops)

@@ -13,7 +13,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libltb.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["ltb_capi.hip", "cheb.hip", "sp2.hip", "apply.hip", "mlp.hip", "batch.hip", "assemble.hip"]
+HIP_SOURCES = ["ltb_capi.hip", "cheb.hip", "sp2.hip", "apply.hip", "mlp.hip", "batch.hip", "assemble.hip", "stochastic.hip"]
 CPP_SOURCES = ["system.cpp", "petsc_lite.cpp", "matlab_io.cpp"]
 DRIVER = os.path.join(HERE, "ltb_scf")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",

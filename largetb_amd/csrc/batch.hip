@@ -7,8 +7,9 @@
 // Layout: atoms of all structures concatenated (BSR with global block
 // columns, block-diagonal across structures).  Structure s keeps its whole
 // exact trace in flight: T_n(s) is norb_s x ld_s (ld_s = norb_s rounded up to
-// 64), row-major, at byte offset 8 xoff[s] (FP32 vectors use half the room).  A task is (s, 64-column group, 16 atoms);
-// a resident grid walks the task list, skipping converged structures.
+// 64), row-major, at byte offset 8 xoff[s] (FP32 vectors use half the room).
+// A task is (s, column group, 4 apw atoms); a resident grid walks the task
+// slots, skipping structures that are done or in the other precision.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

@@ -168,6 +168,43 @@ __global__ __launch_bounds__(256) void k_mom_total(const double* __restrict__ mo
   if (threadIdx.x == 0) tot[m] = sh[0];
 }
 
+// Per-column moments of explicit start vectors t_0 = r_c (stochastic trace,
+// getKPE on given vectors):  mu_0 = |r_c|^2,  mu_1 = <t_1, t_0>,
+//   mu_{2n+1} = 2 <t_{n+1}, t_n> - mu_1,  mu_{2n+2} = 2 |t_{n+1}|^2 - |r_c|^2
+// i.e. r_c^T T_m r_c; momcol is [M+1][ld].  Partial rows as k_cheb_reduce.
+__global__ __launch_bounds__(256) void k_block_reduce(const double* __restrict__ part, int count,
+                                                      int sweep_W, int slots, int ld, int nvalid, int n,
+                                                      int M, double* __restrict__ mu1col,
+                                                      const double* __restrict__ r2col,
+                                                      double* __restrict__ momcol) {
+  const int lane = threadIdx.x & 63, slice = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + lane;
+  const int base = sweep_W > 0 ? ((c / sweep_W) & 7) * slots : 0;
+  double sxy = 0, syy = 0;
+  for (int s = slice; s < count; s += 4) {
+    sxy += part[(size_t)(2 * (base + s)) * ld + c];
+    syy += part[(size_t)(2 * (base + s) + 1) * ld + c];
+  }
+  __shared__ double sh[2][4][64];
+  sh[0][slice][lane] = sxy;
+  sh[1][slice][lane] = syy;
+  __syncthreads();
+  if (slice != 0 || c >= nvalid) return;
+  sxy = (sh[0][0][lane] + sh[0][1][lane]) + (sh[0][2][lane] + sh[0][3][lane]);
+  syy = (sh[1][0][lane] + sh[1][1][lane]) + (sh[1][2][lane] + sh[1][3][lane]);
+  const double r2 = r2col[c];
+  double modd;
+  if (n == 0) {
+    modd = sxy;
+    mu1col[c] = sxy;
+    momcol[c] = r2;
+  } else {
+    modd = 2.0 * sxy - mu1col[c];
+  }
+  if (2 * n + 1 <= M) momcol[(size_t)(2 * n + 1) * ld + c] = modd;
+  if (2 * n + 2 <= M) momcol[(size_t)(2 * n + 2) * ld + c] = 2.0 * syy - r2;
+}
+
 struct Plan {
   int V, W, ld, schedule, nt, order;
   int apw, n_sg;   // blocked
@@ -242,8 +279,10 @@ void launch_mode(const ltb_ctx* c, int mode, const T* hv, const T* xcur, T* xnex
     launch_nt<T, V, 0>(c, hv, xcur, xnext, ldc, j0, nvalid, p);
   else if (mode == 1)
     launch_nt<T, V, 1>(c, hv, xcur, xnext, ldc, j0, nvalid, p);
-  else
+  else if (mode == 2)
     launch_nt<T, V, 2>(c, hv, xcur, xnext, ldc, j0, nvalid, p);
+  else
+    launch_nt<T, V, 3>(c, hv, xcur, xnext, ldc, j0, nvalid, p);
 }
 
 template <typename T>
@@ -387,7 +426,59 @@ int time_steps(ltb_ctx* c, int steps, double* seconds, double* bytes) {
   return LTB_OK;
 }
 
+// Moments r_c^T T_m(H~) r_c, m = 0..M, of nvec explicit vectors (columns of
+// R, row-major [norb][ld] already in d_xa as T, zero beyond nvec) by the
+// same recursion with moment doubling: step 0 is MODE 3 (t_1 = H~ t_0 with
+// t_0 read), then MODE 2 over the ping-pong pair (t_0 stays in d_xa, so
+// step 1 reads it as t_{n-1}).  One chunk of ld = nvec rounded up.
+template <typename T>
+int run_block_moments(ltb_ctx* c, int M, int nvec, const double* d_r2, double* d_momcol, int ld) {
+  Plan p = make_plan<T>(c);
+  p.ld = ld;
+  const int n_cg = ld / p.W;
+  if (p.schedule == LTB_SCHED_SWEEP && n_cg < 8) p.schedule = LTB_SCHED_BLOCKED;
+  if (p.schedule == LTB_SCHED_BLOCKED) {
+    const int target_blocks = 8 * std::max(c->cu_count, 8);
+    const int sg_want = std::max(1, (target_blocks + n_cg - 1) / n_cg);
+    p.apw = std::max(1, (c->nat + 4 * sg_want - 1) / (4 * sg_want));
+    p.n_sg = (c->nat + 4 * p.apw - 1) / (4 * p.apw);
+    p.part_rows = p.n_sg;
+  }
+  int st;
+  if ((st = ensure_buffer(c, (void**)&c->d_part, &c->partbytes, p.part_rows * 2 * (size_t)ld * sizeof(double))))
+    return st;
+  if ((st = ensure_buffer(c, (void**)&c->d_mu1, &c->mu1bytes, (size_t)ld * sizeof(double)))) return st;
+  const T* hv = sizeof(T) == 4 ? (const T*)c->d_ht32 : (const T*)c->d_ht64;
+  T* A = (T*)c->d_xa;
+  T* B = (T*)c->d_xb;
+  const bool sweep = p.schedule == LTB_SCHED_SWEEP;
+  const int nsteps = (M + 1) / 2;
+  for (int n = 0; n < nsteps; n++) {
+    const int mode = n == 0 ? 3 : 2;
+    T* cur = (n & 1) ? B : A;
+    T* nxt = (n & 1) ? A : B;
+    launch_step<T>(c, mode, hv, cur, nxt, ld, 0, nvec, p);
+    LTB_HIP(c, hipGetLastError());
+    hipLaunchKernelGGL(k_block_reduce, dim3(ld / 64), dim3(256), 0, c->stream, c->d_part,
+                       sweep ? p.slots : p.n_sg, sweep ? p.W : 0, p.slots, ld, nvec, n, M, c->d_mu1, d_r2,
+                       d_momcol);
+    LTB_HIP(c, hipGetLastError());
+  }
+  return LTB_OK;
+}
+
 }  // namespace
+
+int kpe_block_moments_impl(ltb_ctx* c, int M, int nvec, int precision, const double* d_r2, double* d_momcol,
+                           int ld) {
+  if (precision == LTB_FP32_ACC64) return run_block_moments<float>(c, M, nvec, d_r2, d_momcol, ld);
+  return run_block_moments<double>(c, M, nvec, d_r2, d_momcol, ld);
+}
+
+int kpe_block_width(ltb_ctx* c, int precision, int nvec) {
+  const int W = precision == LTB_FP32_ACC64 ? make_plan<float>(c).W : make_plan<double>(c).W;
+  return (nvec + W - 1) / W * W;
+}
 
 int kpe_moments_impl(ltb_ctx* c, int M, int precision) {
   if (precision == LTB_FP32_ACC64) return run_moments<float>(c, M);

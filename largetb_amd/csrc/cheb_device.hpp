@@ -40,6 +40,7 @@ __device__ __forceinline__ void store_row(T* p, const T (&x)[V]) {
 // MODE 0: t_1 = H~ t_0 (t_0 implicit identity columns, nothing read)
 // MODE 1: t_2 = 2 H~ t_1 - t_0 (t_0 implicit)
 // MODE 2: t_{n+1} = 2 H~ t_n - t_{n-1}, t_{n-1} read from and overwritten in xnext
+// MODE 3: t_1 = H~ t_0 with t_0 explicit vectors read from xcur
 template <typename T, int V, int MODE, bool NT>
 __device__ __forceinline__ void atom_step(const int* __restrict__ rowptr,
                                           const int* __restrict__ col, const T* __restrict__ hv,
@@ -106,7 +107,7 @@ __device__ __forceinline__ void atom_step(const int* __restrict__ rowptr,
   for (int r = 0; r < 4; r++)
 #pragma unroll
     for (int v = 0; v < V; v++) {
-      if (MODE == 0)
+      if (MODE == 0 || MODE == 3)
         y[r][v] = acc[r][v];
       else if (MODE == 1)
         y[r][v] = T(2) * acc[r][v] - ((4 * Al + r == j0 + c0 + v && c0 + v < nvalid) ? T(1) : T(0));

@@ -84,6 +84,10 @@ void release_system(ltb_ctx* c);
 int alloc_system(ltb_ctx* c, int nat, int nnzb, int gnnz);
 int ensure_buffer(ltb_ctx* c, void** p, size_t* cap, size_t bytes);
 int kpe_moments_impl(ltb_ctx* c, int M, int precision);
+// moments r_c^T T_m r_c of explicit vectors already in d_xa (row-major [norb][ld])
+int kpe_block_moments_impl(ltb_ctx* c, int M, int nvec, int precision, const double* d_r2, double* d_momcol,
+                           int ld);
+int kpe_block_width(ltb_ctx* c, int precision, int nvec);
 int kpe_time_steps(ltb_ctx* c, int precision, int steps, double* seconds, double* bytes);
 int kpe_resolved_tiling(ltb_ctx* c, int precision, ltb_kpe_tiling* out);
 typedef ltb_kpe_tiling KpeTiling;

@@ -142,6 +142,12 @@ __global__ void k_fd_weights(const double* __restrict__ tot, const double* __res
   X[k] = cos(th);
 }
 
+// the K Chebyshev-Gauss nodes alone (k_fd_weights writes them along with W)
+__global__ void k_nodes(int K, double* __restrict__ X) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < K) X[k] = cos(kPi * (k + 0.5) / K);
+}
+
 __device__ double step_count(double mu, const double* g, const double* tot, int M) {
   // getKPE mu_mvals (:1100-1103) times gmMs, summed against the moments
   double am = acos(mu);
@@ -681,6 +687,17 @@ int ltb_get_shift(ltb_ctx* c, double* v) {
   return LTB_OK;
 }
 
+int ltb_set_neutral(ltb_ctx* c, const double* p0) {
+  int st = need_system(c);
+  if (st) return st;
+  if (!p0) return set_err(c, LTB_ERR_ARG, "null neutral populations");
+  for (int i = 0; i < c->nat; i++)
+    if (!std::isfinite(p0[i])) return set_err(c, LTB_ERR_NONFINITE, "non-finite neutral population");
+  LTB_HIP(c, hipMemcpyAsync(c->d_p0, p0, sizeof(double) * c->nat, hipMemcpyHostToDevice, c->stream));
+  LTB_HIP(c, hipStreamSynchronize(c->stream));
+  return LTB_OK;
+}
+
 int ltb_get_mixed_populations(ltb_ctx* c, double* p) {
   int st = need_system(c);
   if (st) return st;
@@ -877,6 +894,25 @@ int ltb_kpe_weights(ltb_ctx* c, int M, double* w) {
   int st = need_system(c);
   if (st) return st;
   if (!w || M != c->fd_M || !c->d_w) return set_err(c, LTB_ERR_STATE, "no ltb_density call with this M");
+  LTB_HIP(c, hipMemcpyAsync(w, c->d_w, sizeof(double) * M, hipMemcpyDeviceToHost, c->stream));
+  LTB_HIP(c, hipStreamSynchronize(c->stream));
+  return LTB_OK;
+}
+
+int ltb_kpe_weights_at(ltb_ctx* c, int M, double mu_ha, double kT, int K, double* w) {
+  int st = need_system(c);
+  if (st) return st;
+  if (!w || M < 2 || !std::isfinite(mu_ha)) return set_err(c, LTB_ERR_ARG, "bad arguments to ltb_kpe_weights_at");
+  if (kT <= 0) K = 1;
+  if (K < 1) return set_err(c, LTB_ERR_ARG, "K >= 1 quadrature nodes needed");
+  if ((st = ensure_fermi(c, M, K))) return st;
+  DevScalars h;
+  if ((st = read_scalars(c, &h))) return st;
+  h.mu = (mu_ha - h.b) / h.a;  // the H~ frame of the last ltb_rescale
+  LTB_HIP(c, hipMemcpyAsync(c->d_sc, &h, sizeof(DevScalars), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_nodes, dim3(grid_for(K, 256)), dim3(256), 0, c->stream, K, c->d_X);
+  hipLaunchKernelGGL(k_coeffs, dim3(M), dim3(256), 0, c->stream, c->d_X, K, c->d_g, M, kT, c->d_sc, c->d_w);
+  LTB_HIP(c, hipGetLastError());
   LTB_HIP(c, hipMemcpyAsync(w, c->d_w, sizeof(double) * M, hipMemcpyDeviceToHost, c->stream));
   LTB_HIP(c, hipStreamSynchronize(c->stream));
   return LTB_OK;

@@ -46,6 +46,45 @@ int diag_block(State& s, int depth, int j0, int j1, std::vector<double>& out) {
 
 std::string dump_path(const State& s, const char* file) { return s.opt.dump_dir + "/" + file; }
 
+// getKPE's coefficients c_m(mu) g_m (:1094-1132): from the bisected mu of the
+// exact mode, or at runKPM's DOS mu in stochastic mode
+int kpe_weights(State& s, std::vector<double>& w) {
+  w.assign(s.opt.moments, 0.0);
+  if (s.opt.nrandoms > 0) {
+    const double kT = s.opt.kT_ev > 0 ? s.opt.kT_ev / 27.211386 : 0.0;
+    CHK(s, ltb_kpe_weights_at(s.ctx, s.opt.moments, s.mu_ha, kT, s.opt.quad_nodes, w.data()));
+  } else {
+    CHK(s, ltb_kpe_weights(s.ctx, s.opt.moments, w.data()));
+  }
+  return LTB_OK;
+}
+
+// computeDiagonalElements' random mode (:1338-1392) at SP2 depth `depth`:
+// diag = sum_r r.*(P_depth r) ./ sum_r r.*r over opt.nrandoms seeded vectors
+int random_diag(State& s, int depth, std::vector<double>& out) {
+  const int nR = s.opt.nrandoms, n = s.nSize;
+  const unsigned long long seed = s.opt.seed + 1000003ULL * ++s.draws;
+  out.assign(n, 0.0);
+  if (depth == 0) {
+    std::vector<double> w;
+    CHK(s, kpe_weights(s, w));
+    CHK(s, ltb_density_stochastic(s.ctx, s.opt.moments, w.data(), nR, seed, s.opt.precision, out.data()));
+    return LTB_OK;
+  }
+  std::vector<double> R((size_t)nR * n), num(n, 0.0), den(n, 0.0);
+  if (ltb_random_vectors(seed, nR, n, R.data()) != LTB_OK) return LTB_ERR_ARG;
+  for (int k = 0; k < nR; k++) {
+    Vec r(R.begin() + (size_t)k * n, R.begin() + (size_t)(k + 1) * n), y;
+    CHK(s, getSP2(s, r, y, depth, depth));
+    for (int j = 0; j < n; j++) {
+      num[j] += r[j] * y[j];
+      den[j] += r[j] * r[j];
+    }
+  }
+  for (int j = 0; j < n; j++) out[j] = num[j] / den[j];
+  return LTB_OK;
+}
+
 }  // namespace
 
 PetscErrorCode arrangeCoordinates(State& s) {
@@ -58,6 +97,12 @@ PetscErrorCode arrangeCoordinates(State& s) {
   s.nSize = 4 * s.Natoms;
   s.neutralCharges.assign(s.Natoms, 0.0);
   CHK(s, ltb_system_neutral(s.sys, s.neutralCharges.data()));
+  if (s.opt.reference_literal) {  // localAtoms == 8 has no branch at :714-720
+    std::vector<int> Z(s.Natoms);
+    CHK(s, ltb_system_coords(s.sys, Z.data(), nullptr));
+    for (int i = 0; i < s.Natoms; i++)
+      if (Z[i] == 8) s.neutralCharges[i] = 0.0;
+  }
   s.chargeOnAtom = s.neutralCharges;
   s.mixedCharges = s.neutralCharges;
   s.previousCharges = s.neutralCharges;
@@ -68,6 +113,7 @@ PetscErrorCode initializeForH(State& s) {
   if (!s.sys) return LTB_ERR_STATE;
   if (!s.ctx) CHK(s, ltb_create(s.opt.device, &s.ctx));
   CHK(s, ltb_load_system(s.ctx, s.sys));
+  if (s.opt.reference_literal) CHK(s, ltb_set_neutral(s.ctx, s.neutralCharges.data()));
   CHK(s, ltb_set_kpe_orbital_moments(s.ctx, 1));
   return LTB_OK;
 }
@@ -113,19 +159,44 @@ PetscErrorCode GershGorinParallel(State& s, double* mn, double* mx) {
 }
 
 PetscErrorCode runKPM(State& s) {
-  // bounds and rescale as :1541-1560; mu by FP64 bisection on the exact
-  // moments in place of the stochastic DOS search (:1562-1712)
+  // bounds and rescale as :1541-1560
   double mn, mx;
   CHK(s, GershGorinParallel(s, &mn, &mx));
   s.a = (mx - mn) / 2;
   s.b = (mx + mn) / 2;
   CHK(s, ltb_rescale(s.ctx, s.a, s.b));
+  if (s.opt.nrandoms > 0) {
+    // stochastic moments of the trace, DOS and mu from its area (:1562-1701).
+    // The reference then narrows [a, b] to where the DOS exceeds 0.001 per
+    // orbital (:1702-1712); that can cut the band edges, and T_m of an
+    // eigenvalue outside [-1, 1] grows like cosh(m acosh|x|), so the
+    // Gershgorin frame is kept and the narrowed bounds are only reported.
+    const int nm = s.opt.dos_moments;
+    std::vector<double> tr(nm);
+    CHK(s, ltb_kpe_stochastic(s.ctx, nm - 1, s.opt.nrandoms, s.opt.seed + 1000003ULL * ++s.draws, s.opt.precision,
+                              tr.data()));
+    s.dosEnergies.assign(2 * nm, 0.0);
+    s.dosValues.assign(2 * nm, 0.0);
+    double mu, lo, hi;
+    CHK(s, ltb_kpm_dos(tr.data(), nm, 2 * nm, s.a, s.b, s.nSize, s.Nelectrons, s.dosEnergies.data(),
+                       s.dosValues.data(), &mu, &lo, &hi));
+    if (!s.opt.dump_dir.empty())
+      CHKIO(s, ltb_write_dos(dump_path(s, "dosfile.txt").c_str(), 2 * nm, s.dosEnergies.data(),
+                             s.dosValues.data()));
+    s.mu_ha = mu;
+    s.muval = (mu - s.b) / s.a;
+    s.eband = 0;
+    return LTB_OK;
+  }
+  // exact trace: mu by FP64 bisection on the exact moments in place of the
+  // stochastic DOS search
   CHK(s, ltb_kpe_moments(s.ctx, s.opt.moments, s.opt.precision, nullptr, nullptr));
   const double kT = s.opt.kT_ev > 0 ? s.opt.kT_ev / 27.211386 : 0.0;
   ltb_fermi_params fp{s.opt.moments, kT, s.opt.quad_nodes, s.Nelectrons};
   ltb_density_result r;
   CHK(s, ltb_density(s.ctx, &fp, nullptr, &r));
   s.muval = r.mu;
+  s.mu_ha = r.mu_ha;
   s.eband = r.eband;
   return LTB_OK;
 }
@@ -133,7 +204,7 @@ PetscErrorCode runKPM(State& s) {
 PetscErrorCode getKPE(State& s, const Vec& in, Vec& out) {
   if ((int)in.size() != s.nSize) return LTB_ERR_ARG;
   std::vector<double> w(s.opt.moments);
-  CHK(s, ltb_kpe_weights(s.ctx, s.opt.moments, w.data()));
+  CHK(s, kpe_weights(s, w));
   out.assign(s.nSize, 0.0);
   CHK(s, ltb_kpe_apply(s.ctx, s.opt.moments, w.data(), 1, in.data(), out.data(), LTB_FP64));
   return LTB_OK;
@@ -161,15 +232,16 @@ PetscErrorCode computeAlphas(State& s) {
   }
   for (int k = 0; k < s.opt.maxsp2; k++) {
     double tr = 0;
-    if (k == 0) {
+    if (s.opt.nrandoms > 0) {
+      // the random-vector trace estimate of the level (:1240-1275)
+      std::vector<double> d;
+      CHK(s, random_diag(s, k, d));
+      for (double x : d) tr += x;
+    } else if (k == 0) {
       // Tr P_KPE from the exact moments (the exact unit-vector trace)
-      double ne = 0;
-      std::vector<double> w(s.opt.moments);
-      CHK(s, ltb_kpe_weights(s.ctx, s.opt.moments, w.data()));
       std::vector<double> d(s.nSize);
       CHK(s, ltb_density_diag(s.ctx, d.data()));
-      for (double x : d) ne += x;
-      tr = ne;
+      for (double x : d) tr += x;
     } else {
       std::vector<double> d(s.nSize);
       CHK(s, diag_block(s, k, 0, s.nSize, d));
@@ -185,7 +257,9 @@ PetscErrorCode computeAlphas(State& s) {
 PetscErrorCode computeDiagonalElements(State& s) {
   const int depth = s.opt.maxsp2 - 1;
   s.diagP.assign(s.nSize, 0.0);
-  if (depth == 0)
+  if (s.opt.nrandoms > 0)
+    CHK(s, random_diag(s, depth, s.diagP));
+  else if (depth == 0)
     CHK(s, ltb_density_diag(s.ctx, s.diagP.data()));
   else
     CHK(s, diag_block(s, depth, 0, s.nSize, s.diagP));
@@ -210,6 +284,17 @@ PetscErrorCode constructdeltaH(State& s) {
   CHK(s, ltb_set_populations(s.ctx, s.mixedCharges.data()));
   s.gammaq.assign(s.Natoms, 0.0);
   CHK(s, ltb_get_shift(s.ctx, s.gammaq.data()));
+  if (s.opt.reference_literal) {
+    // the reference forms Delta H from the latest populations (:1418-1420) and
+    // mixes Hamiltonians afterwards; report that Delta q and Gamma Delta q
+    for (int i = 0; i < s.Natoms; i++) s.netCharges[i] = s.chargeOnAtom[i] - s.neutralCharges[i];
+    for (int i = 0; i < s.Natoms; i++) {
+      double g = 0;
+      for (int k = s.Gamma_ab.rowptr[i]; k < s.Gamma_ab.rowptr[i + 1]; k++)
+        g += s.Gamma_ab.val[k] * s.netCharges[s.Gamma_ab.col[k]];
+      s.gammaq[i] = g;
+    }
+  }
   if (!s.opt.dump_dir.empty()) {
     // deltaH as 4x4 diagonal blocks, netCharges and gammaq (:1465-1484)
     std::vector<int> rp(s.Natoms + 1), col(s.Natoms);
@@ -236,7 +321,37 @@ PetscErrorCode updateHtotal(State& s) {
   return LTB_OK;
 }
 
+// The reference's loop (main :1805-1840): one density of H before the loop,
+// then per iteration Delta H, Hamiltonian mixing, runKPM, density, and the
+// change |q_k - q_(k-1)| of successive outputs.
+PetscErrorCode scf_literal(State& s, int* iterations, double* delta) {
+  CHK(s, constructdeltaH(s));
+  CHK(s, runKPM(s));
+  CHK(s, computeDensityMatrix(s));
+  s.previousCharges = s.chargeOnAtom;
+  int it = 0;
+  double d = INFINITY;
+  for (it = 0; it < s.opt.max_scf; it++) {
+    CHK(s, updateHtotal(s));
+    CHK(s, constructdeltaH(s));
+    CHK(s, runKPM(s));
+    CHK(s, computeDensityMatrix(s));
+    d = 0;
+    for (int i = 0; i < s.Natoms; i++) d = std::max(d, 2.0 * std::fabs(s.chargeOnAtom[i] - s.previousCharges[i]));
+    if (s.opt.verbose) std::printf("scfiter : %d , the value of the scfDelta is %f \n", it, d);  // :1834
+    s.previousCharges = s.chargeOnAtom;
+    if (d < s.opt.tol) {
+      it++;
+      break;
+    }
+  }
+  if (iterations) *iterations = it;
+  if (delta) *delta = d;
+  return d < s.opt.tol ? LTB_OK : LTB_NOT_CONVERGED;
+}
+
 PetscErrorCode scf(State& s, int* iterations, double* delta) {
+  if (s.opt.reference_literal) return scf_literal(s, iterations, delta);
   int it = 0;
   double d = INFINITY;
   for (it = 0; it < s.opt.max_scf; it++) {

@@ -1,8 +1,10 @@
 // ltb_scf: the reference's driver (`./scf L M N nRandoms`, withSP2_deltaH.cpp
-// main :1742-1849) over the entry points of ltb_petsc.hpp.  The trace is exact
-// here, so the 4th argument of the reference (nRandoms) is not taken.
-//   ltb_scf L M N [--sigma A] [--seed S] [--moments M] [--kT eV] [--reference]
+// main :1742-1849) over the entry points of ltb_petsc.hpp.  Without nRandoms
+// the trace is exact over unit vectors; with it, runKPM's stochastic DOS and
+// mu and the random-vector diagonal are used, as in the reference.
+//   ltb_scf L M N [nRandoms] [--sigma A] [--seed S] [--moments M] [--kT eV] [--reference]
 //                 [--maxsp2 D] [--tol Q] [--max-scf K] [--fp32] [--sp2-alpha a] [--dump DIR]
+//                 [--mix m] [--reference-literal]
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -19,15 +21,20 @@ static double now() {
 
 int main(int argc, char** argv) {
   if (argc < 4) {
-    std::fprintf(stderr, "usage: %s L M N [--sigma A] [--seed S] [--moments M] [--kT eV] "
-                 "[--reference] [--maxsp2 D] [--tol Q] [--max-scf K] [--fp32] [--sp2-alpha a] [--dump DIR]\n", argv[0]);
+    std::fprintf(stderr, "usage: %s L M N [nRandoms] [--sigma A] [--seed S] [--moments M] [--kT eV] "
+                 "[--reference] [--maxsp2 D] [--tol Q] [--max-scf K] [--fp32] [--sp2-alpha a] [--dump DIR] [--mix m] [--reference-literal]\n", argv[0]);
     return 2;
   }
   State s;
   s.opt.L = std::atoi(argv[1]);
   s.opt.M = std::atoi(argv[2]);
   s.opt.N = std::atoi(argv[3]);
-  for (int i = 4; i < argc; i++) {
+  int first_opt = 4;
+  if (argc > 4 && argv[4][0] != '-') {  // ./scf L M N nRandoms (:1747-1757)
+    s.opt.nrandoms = std::atoi(argv[4]);
+    first_opt = 5;
+  }
+  for (int i = first_opt; i < argc; i++) {
     std::string a = argv[i];
     auto val = [&]() { return i + 1 < argc ? argv[++i] : (char*)"0"; };
     if (a == "--sigma") s.opt.sigma_ang = std::atof(val());
@@ -41,6 +48,8 @@ int main(int argc, char** argv) {
     else if (a == "--fp32") s.opt.precision = LTB_FP32_ACC64;
     else if (a == "--sp2-alpha") s.opt.force_alpha = std::atof(val());
     else if (a == "--dump") s.opt.dump_dir = val();
+    else if (a == "--mix") s.opt.mix = std::atof(val());
+    else if (a == "--reference-literal") s.opt.reference_literal = true;
     else {
       std::fprintf(stderr, "unknown option %s\n", a.c_str());
       return 2;
@@ -73,7 +82,7 @@ int main(int argc, char** argv) {
   std::printf("{\"status\": %d, \"atoms\": %d, \"iterations\": %d, \"delta\": %.3e, \"seconds\": %.3f, "
               "\"mu_ha\": %.12f, \"eband_ha\": %.12f, \"total_charge\": %.12f, "
               "\"q_si\": [%.9f, %.9f], \"q_o\": [%.9f, %.9f]}\n",
-              st, s.Natoms, iters, delta, t_scf, s.a * s.muval + s.b, s.eband, tot, qsi_min, qsi_max,
+              st, s.Natoms, iters, delta, t_scf, s.mu_ha, s.eband, tot, qsi_min, qsi_max,
               qo_min, qo_max);
   // full charge vector for checkers
   if (const char* path = std::getenv("LTB_SCF_CHARGES")) {

@@ -553,4 +553,9 @@ Status locality_order(const ltb_system& s, std::vector<int>& order) {
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
   return Status::ok();
 }
+void gaussian_fill(uint64_t seed, size_t n, double* out) {
+  Gauss g{seed};
+  for (size_t k = 0; k < n; k++) out[k] = g.normal();
+}
+
 }  // namespace ltb

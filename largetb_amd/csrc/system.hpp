@@ -47,4 +47,8 @@ Status pairs_within(const ltb_system& s, double rc, std::vector<int>& rowptr,
 // Visiting order of the atoms that keeps bonded atoms close in sequence
 // (strips of spatial bins); a permutation of 0..natoms-1.
 Status locality_order(const ltb_system& s, std::vector<int>& order);
+
+// n standard normal numbers of the SplitMix64 -> Box-Muller stream of `seed`
+// (the generator of the structure displacements)
+void gaussian_fill(uint64_t seed, size_t n, double* out);
 }  // namespace ltb

@@ -113,15 +113,20 @@ def assemble(ctx, structures, q, offs, status):
                 seg=np.array(seg, np.int32), qtot=np.array(qtot))
 
 
-def train(ctx, data, steps=3000, lr=2e-3, seed=1, log=None):
+def train(ctx, data, steps=3000, lr=2e-3, seed=1, log=None, seconds=None):
+    """Full-batch Adam on the device; lr drops 4x at half the steps (or half
+    of `seconds`).  With `seconds`, training stops at the first 100-step
+    boundary past that time."""
     m = Mlp(ctx, seed=seed)
     m.set_data(data["feat"], data["Z"], data["seg"], data["qtot"], data["q"])
     m.fit_normalization()
     hist = []
     done = 0
-    while done < steps:
+    t0 = time.perf_counter()
+    while done < steps and (seconds is None or time.perf_counter() - t0 < seconds):
         k = min(100, steps - done)
-        h = m.adam(k, lr=lr if done < steps // 2 else lr / 4)
+        late = done >= steps // 2 or (seconds is not None and time.perf_counter() - t0 > seconds / 2)
+        h = m.adam(k, lr=lr / 4 if late else lr)
         hist.extend(h.tolist())
         done += k
         if log:
@@ -156,6 +161,7 @@ def main(argv=None):
     ap.add_argument("--M", type=int, default=512)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--train-seconds", type=float, default=None)
     ap.add_argument("--batched", action="store_true")
     ap.add_argument("--charges-out", default="")
     ap.add_argument("--charges-in", nargs="*", default=[])
@@ -206,11 +212,11 @@ def main(argv=None):
         np.savez_compressed(a.data_out, feat=data["feat"], Z=data["Z"], q=data["q"], seg=data["seg"],
                             qtot=data["qtot"])
     t0 = time.perf_counter()
-    m, hist = train(ctx, data, steps=a.steps, lr=a.lr, log=log)
+    m, hist = train(ctx, data, steps=a.steps, lr=a.lr, log=log, seconds=a.train_seconds)
     ttrain = time.perf_counter() - t0
     q = m.forward()
     err = np.abs(q - data["q"])
-    info = dict(count=len(data["seg"]) - 1, seed=a.seed, M=M, atoms=int(len(data["Z"])), steps=a.steps,
+    info = dict(count=len(data["seg"]) - 1, seed=a.seed, M=M, atoms=int(len(data["Z"])), steps=len(hist),
                 train_mse=float(hist[-1]), train_max_err=float(err.max()),
                 generate_s=round(tgen, 1), train_s=round(ttrain, 1),
                 mean_scc_iters=float(np.mean(iters)))

@@ -1,7 +1,17 @@
-"""ctypes front end of the CPU FP64 oracle (oracle.c).
+"""ctypes front end of the CPU FP64 oracle (oracle.c), plus two numpy
+restatements for the stochastic mode.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
 bench.py's cpu_baseline leg, never by the product package.
+
+Pins of the numpy functions (the C functions' pins are in oracle.c):
+* kpm_dos (runKPM :1634-1701): its Jackson kernel is `jackson`, pinned by
+  the reference's gnsfile.txt; on exact trace moments of a 72-atom cell the DOS
+  integrates to the orbital count, mu falls in the HOMO-LUMO gap of the dense
+  eigenvalues and the narrowed bounds sit at the extreme eigenvalues
+  (tests/test_stochastic_host.py).
+* gaussian_stream: reproduces the atom displacements that orc_quartz draws
+  (displacement / sigma, 1e-12).
 """
 import ctypes as C
 import os
@@ -297,3 +307,64 @@ def sp2_dense(X, nocc, maxit=100, idem_tol=1e-12):
     if it < 0:
         raise OracleError("orc_sp2_dense failed")
     return X, it, idem.value
+
+
+def kpm_dos(moments, a, b, nsize, nelectrons, ntilde=None):
+    """runKPM's DOS, mu and bound tightening (withSP2_deltaH.cpp:1634-1701),
+    literally: Jackson g_n with N = len(moments), DOS[k] = (g_0 mu_0 + 2 sum_j
+    g_j mu_j cos(j acos x_k)) / (pi sqrt(1 - x_k^2)) at x_k = cos(pi (k + 1/2) /
+    ntilde), trapezoid area from the bottom, mu at area fraction Ne / (2 nsize),
+    bounds where DOS / nsize >= 0.001."""
+    mom = np.asarray(moments, np.float64)
+    n = len(mom)
+    nt = 2 * n if ntilde is None else ntilde
+    g = jackson(n)
+    x = np.cos(np.pi * (np.arange(nt) + 0.5) / nt)
+    j = np.arange(1, n)
+    dos = np.array([(g[0] * mom[0] + 2 * np.sum(g[1:] * mom[1:] * np.cos(j * np.arccos(xv))))
+                    / (np.pi * np.sqrt(1.0 - xv * xv)) for xv in x])
+    E = x * a + b
+    seg = (x[:-1] - x[1:]) * (dos[1:] + dos[:-1]) / 2   # seg[i-1] = trapezoid between x[i-1] and x[i]
+    total = seg[::-1].sum()
+    frac = nelectrons * total / (2.0 * nsize)
+    acc, imu = 0.0, 0
+    for i in range(nt - 1, 0, -1):
+        acc += seg[i - 1]
+        if acc >= frac:
+            imu = i
+            break
+    lo, hi = -10.0, 10.0
+    for i in range(nt - 1, 0, -1):
+        if dos[i] / nsize >= 0.001:
+            lo = E[i]
+            break
+    for i in range(nt):
+        if dos[i] / nsize >= 0.001:
+            hi = E[i]
+            break
+    return dict(energies=E, dos=dos, mu=E[imu], emin=lo, emax=hi, area=total)
+
+
+def gaussian_stream(seed, n):
+    """n standard normals of the SplitMix64 -> Box-Muller stream (uniforms
+    ((z >> 11) + 1) 2^-53, pairs r cos t then r sin t)."""
+    mask = (1 << 64) - 1
+    s = seed & mask
+    out = np.empty(n)
+    k = 0
+    while k < n:
+        u = []
+        for _ in range(2):
+            s = (s + 0x9E3779B97F4A7C15) & mask
+            z = s
+            z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask
+            z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & mask
+            z ^= z >> 31
+            u.append(((z >> 11) + 1) * 2.0 ** -53)
+        r = np.sqrt(-2.0 * np.log(u[0]))
+        t = 2.0 * np.pi * u[1]
+        out[k] = r * np.cos(t)
+        if k + 1 < n:
+            out[k + 1] = r * np.sin(t)
+        k += 2
+    return out

@@ -103,3 +103,50 @@ def test_driver_writes_the_reference_dumps(tmp_path, ref_dumps):
     D = np.zeros(shape)
     D[rr, cc] = vv
     assert np.array_equal(D, np.diag(np.repeat(gq, 4)))
+
+
+@pytest.mark.gpu
+def test_driver_stochastic_mode(tmp_path):
+    """`ltb_scf L M N nRandoms`: runKPM's stochastic DOS and mu, and the
+    random-vector diagonal.  On the neutral first iteration mu (the DOS-area
+    rule puts it at the valence-band edge) lands at the gap of H; the charges
+    scatter, by the estimator's noise, around the exact populations at that
+    same mu; dosfile.txt holds the 2 x 1000-point DOS."""
+    d = tmp_path / "dump"
+    d.mkdir()
+    r, out, q = run(["2", "2", "2", "64", "--sigma", "0.05", "--seed", "1", "--max-scf", "1", "--dump", str(d)],
+                    tmp_path)
+    assert r.returncode in (0, 5), r.stderr
+    o = O.System.quartz(2, 2, 2, sigma_ang=0.05, seed=1)
+    e = np.linalg.eigvalsh(o.dense_h())
+    assert e[191] - 0.02 < out["mu_ha"] < e[192] + 0.02
+    # exact populations at the driver's mu: per-atom moments times the weights
+    lo, hi = o.gershgorin()
+    a, b = (hi - lo) / 2, (hi + lo) / 2
+    c = O.fd_coeffs(200, (out["mu_ha"] - b) / a, 0.05 / O.HARTREE_EV / a, 8192)
+    w = O.kpe_weights(c, O.gmM(200))
+    exact = 2 * (o.diag_moments(a, b, 200) @ w).reshape(-1, 4).sum(1)
+    assert np.abs(q - exact).mean() < 0.3 and np.abs(q - exact).max() < 2.0
+    assert abs(q.sum() - exact.sum()) < 12.0
+    assert len((d / "dosfile.txt").read_text().splitlines()) == 2000
+
+
+@pytest.mark.gpu
+def test_driver_reference_literal_mode(tmp_path):
+    """--reference-literal with mixRatio 0 (the reference's default, :56):
+    H_total never changes, so the loop stops on its first iteration with a
+    zero change between successive outputs, the charges are the populations
+    of H itself, and netCharge.m holds them minus the literal neutral charges
+    (Si 2, O 0 per spin, :714-720)."""
+    import largetb_amd as L
+    d = tmp_path / "dump"
+    d.mkdir()
+    r, out, q = run(["2", "2", "2", "--sigma", "0.05", "--seed", "1", "--reference-literal", "--mix", "0",
+                     "--tol", "0.1", "--max-scf", "20", "--dump", str(d)], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert out["iterations"] == 1 and out["delta"] < 1e-12
+    o = O.System.quartz(2, 2, 2, sigma_ang=0.05, seed=1)
+    ref = o.density(200)["pop"]
+    assert np.abs(q - 2 * ref).max() < 1e-8
+    net = L.read_matlab_vec(d / "netCharge.m")
+    assert np.abs(net - (ref - np.where(o.Z == 14, 2.0, 0.0))).max() < 1e-8
