@@ -106,7 +106,7 @@ def time_to_solution(args):
     s = L.System.quartz(*args.cells, sigma_ang=args.sigma, seed=args.seed)
     ctx = L.Context(0)
     ctx.load(s)
-    m, info = learned.load(ctx)
+    m, info = learned.load(ctx, args.weights) if args.weights else learned.load(ctx)
     log(f"weights: {info}")
     # tilings for both precisions, and a short warm-up of both code paths
     ctx.set_populations(None)
@@ -279,7 +279,7 @@ def scc_segment(args):
     if st is not None:
         guess = st["pop"]
     elif args.start == "learned":
-        m, info = learned.load(ctx)
+        m, info = learned.load(ctx, args.weights) if args.weights else learned.load(ctx)
         t0 = time.perf_counter()
         guess = m.predict(s)
         extra = time.perf_counter() - t0
@@ -366,6 +366,7 @@ def main():
     ap.add_argument("--tiling", type=str, default="",
                     help="explicit tiling (implies --no-autotune): chunk,V,schedule,blocks_per_cu,nt[,order]")
     ap.add_argument("--no-ablation", action="store_true")
+    ap.add_argument("--weights", default="", help="--tts / --segment: learned-guess weights (.npz) to use")
     ap.add_argument("--sp2", action="store_true", help="dense SP2 measurement (8x8x8 by default)")
     ap.add_argument("--tts", action="store_true",
                     help="SCC time-to-solution: plain FP64 neutral start vs learned guess + mixed")
@@ -379,7 +380,7 @@ def main():
                     help="one resumable segment of an SCC solve (plain FP64 or FP32 bulk + FP64 finish)")
     ap.add_argument("--start", choices=["neutral", "learned"], default="neutral")
     ap.add_argument("--state", default="", help="segment: state file to resume from")
-    ap.add_argument("--state-out", default="gpurun_out/scc_state.npz")
+    ap.add_argument("--state-out", default="scc_state.npz")
     ap.add_argument("--max-iter", type=int, default=400)
     ap.add_argument("--compare", nargs=2, metavar=("PLAIN", "FAST"),
                     help="errors and speed-up between two finished segment state files")

@@ -339,7 +339,8 @@ int ltb_kpe_autotune(ltb_ctx* ctx, int precision, int trial_steps, ltb_kpe_tilin
                      double* best_gbs);
 
 /* Tiling of a batch's step kernel.  Column groups are fixed (256 columns, 4
- * per lane in FP32 and 2 x 2 in FP64, then 64-column groups for the rest);
+ * per lane in FP32 and 2 x 2 in FP64, then one 64-, 128- or 256-wide group
+ * for the columns left);
  * atoms_per_wave, blocks_per_cu and nontemporal are taken (0 / -1 keep the
  * current value).  No reference counterpart: the reference has one structure
  * per job and no tiling. */
@@ -355,7 +356,7 @@ int ltb_batch_autotune(ltb_batch* b, int precision, int trial_steps, ltb_kpe_til
  * No reference counterpart: the reference starts every SCC from the neutral
  * charges (arrangeCoordinates :714-720).  A per-species MLP on local
  * descriptors predicts converged charges; the plain neutral start remains the
- * fallback and the judge of the answer. */
+ * fallback and the answer the guessed runs are checked against. */
 #define LTB_MLP_NFEAT 80 /* 2 x 32 radial Gaussians (per neighbour species) + 16 angular */
 typedef struct ltb_mlp ltb_mlp;
 int ltb_mlp_num_params(void);

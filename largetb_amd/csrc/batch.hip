@@ -7,7 +7,8 @@
 // Layout: atoms of all structures concatenated (BSR with global block
 // columns, block-diagonal across structures).  Structure s keeps its whole
 // exact trace in flight: T_n(s) is norb_s x ld_s (ld_s = norb_s rounded up to
-// 64), row-major, at byte offset 8 xoff[s] (FP32 vectors use half the room).
+// its column groups), row-major, at byte offset 8 xoff[s] (FP32 vectors use
+// half the room).
 // A task is (s, column group, 4 apw atoms); a resident grid walks the task
 // slots, skipping structures that are done or in the other precision.
 #include <hip/hip_runtime.h>
@@ -28,10 +29,17 @@ namespace {
 constexpr double kPi = 3.141592653589793238463;
 enum { S_EMIN, S_EMAX, S_A, S_B, S_MU, S_NELEC, S_EBAND, S_EBFP, S_DQ, S_NE, S_BAD, NSC };
 
+// Column groups of a structure: 256 wide (FP32: 4 columns per lane, 16-byte
+// accesses; FP64: two passes of 2 per lane) and one tail group for the
+// norb mod 256 columns left, 64, 128 or 256 wide (the narrowest that holds
+// them; padding columns carry zeros).  ld_s = the columns of all groups.
+__host__ __device__ inline int tail_width(int norb) {
+  const int r = norb & 255;
+  return r == 0 ? 0 : (r > 128 ? 256 : (r > 64 ? 128 : 64));
+}
+
 // One task = one column group of one structure x 4*apw consecutive atoms.
-// Wide groups are 256 columns (FP32: 4 columns per lane, 16-byte accesses;
-// FP64: two passes of 2 columns per lane), the structure's last columns form
-// narrow 64-column groups.  Partials: 2 x 256 doubles per task.
+// Partials: 2 x 256 doubles per task.
 // active[s]: 0 done, 1 FP32 recursion, 2 FP64 recursion; the FP32 and FP64
 // instances of this kernel each take the structures of their own precision.
 // slots[]: task per grid-stride slot (-1 = padding), laid out so that slot i
@@ -70,7 +78,8 @@ __global__ __launch_bounds__(256) void k_batch_step(
     T* xn = reinterpret_cast<T*>(xnext + xoffb[s]);
     const int ab = min((sg * 4 + wave) * apw, nat), ae = min(ab + apw, nat);
     const size_t row2 = (size_t)t * 2;
-    if (cstart + 256 <= (nvalid & ~255)) {
+    const int width = cstart < (nvalid & ~255) ? 256 : tail_width(nvalid);
+    if (width == 256) {
       if constexpr (sizeof(T) == 4) {
         task_pass<T, 4, MODE, NT>(rowptr, col, hv, xc, xn, a0, ab, ae, ld, cstart + 4 * lane, nvalid, wave,
                                   lane, part, row2, 0);
@@ -79,6 +88,9 @@ __global__ __launch_bounds__(256) void k_batch_step(
           task_pass<T, 2, MODE, NT>(rowptr, col, hv, xc, xn, a0, ab, ae, ld, cstart + 128 * h + 2 * lane,
                                     nvalid, wave, lane, part, row2, 128 * h);
       }
+    } else if (width == 128) {
+      task_pass<T, 2, MODE, NT>(rowptr, col, hv, xc, xn, a0, ab, ae, ld, cstart + 2 * lane, nvalid, wave, lane,
+                                part, row2, 0);
     } else {
       task_pass<T, 1, MODE, NT>(rowptr, col, hv, xc, xn, a0, ab, ae, ld, cstart + lane, nvalid, wave, lane,
                                 part, row2, 0);
@@ -107,10 +119,10 @@ __global__ __launch_bounds__(256) void k_batch_reduce(
   if (use) {
     const int nat = aoff[s + 1] - aoff[s];
     valid = c < 4 * nat;
-    // wide 256-column groups first, then narrow 64-column groups
+    // 256-wide groups first, then the one tail group
     const int cw = (4 * nat) & ~255;
-    const int gidx = c < cw ? c >> 8 : (cw >> 8) + ((c - cw) >> 6);
-    const int off = c < cw ? c & 255 : (c - cw) & 63;
+    const int gidx = c < cw ? c >> 8 : cw >> 8;
+    const int off = c < cw ? c & 255 : c - cw;
     const int base = tbegin[s] + gidx * nsg[s];
     for (int g = 0; g < nsg[s]; g++) {
       sxy += part[(size_t)(base + g) * 512 + off];
@@ -480,7 +492,7 @@ int upload_tmp(ltb_batch* b, int** p, const std::vector<int>& h) {
 }
 
 // Tasks for the batch's tiling: per structure its column groups (256-wide,
-// then 64-wide for the columns left), each split into blocks of
+// then the tail group), each split into blocks of
 // 4*atoms_per_wave atoms.  Column groups go to the least loaded XCD (load =
 // columns x atoms) so that each XCD's L2 serves the gathers of its own
 // groups; slot 8 i + x holds XCD x's i-th task.
@@ -491,11 +503,11 @@ int build_tasks(ltb_batch* b) {
   std::vector<double> xcd_load(8, 0.0);
   for (int s = 0; s < b->S; s++) {
     const int nat = b->aoff[s + 1] - b->aoff[s], norb = 4 * nat, ld = b->lds[s];
-    const int ng = (nat + 4 * apw - 1) / (4 * apw), cw = norb & ~255;
+    const int ng = (nat + 4 * apw - 1) / (4 * apw), cw = norb & ~255, tw = tail_width(norb);
     tbegin.push_back((int)task_s.size());
     nsg.push_back(ng);
     for (int c0 = 0; c0 < ld;) {
-      const int width = c0 < cw ? 256 : 64;
+      const int width = c0 < cw ? 256 : tw;
       const int x = (int)(std::min_element(xcd_load.begin(), xcd_load.end()) - xcd_load.begin());
       xcd_load[x] += (double)width * nat;
       for (int g = 0; g < ng; g++) {
@@ -681,7 +693,7 @@ int ltb_batch_create(ltb_ctx* c, int nstruct, const ltb_system* const* systems, 
   std::vector<int> gc_s, gc_c;
   b->xoff.push_back(0);
   for (int s = 0; s < nstruct; s++) {
-    const int nat = b->aoff[s + 1] - b->aoff[s], norb = 4 * nat, ld = (norb + 63) / 64 * 64;
+    const int nat = b->aoff[s + 1] - b->aoff[s], norb = 4 * nat, ld = (norb & ~255) + tail_width(norb);
     b->lds.push_back(ld);
     b->xoff.push_back(b->xoff.back() + (size_t)norb * ld);
     for (int cc = 0; cc < ld; cc++) {

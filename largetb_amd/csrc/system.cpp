@@ -410,7 +410,7 @@ Status pairs_within(const ltb_system& s, double rc, std::vector<int>& rowptr,
       reach[d] = (int)std::ceil(rc / w) + 1;  // positions may sit anywhere in the cell
       if (w <= 2.0 * rc) minimum_image = false;
     }
-  if (minimum_image && n > 2000) {
+  if (minimum_image && n > 64) {
     // binned search, one image per pair (every periodic width exceeds 2 rc)
     std::vector<double> f(3 * (size_t)n);
     for (int i = 0; i < n; i++)

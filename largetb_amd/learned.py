@@ -96,10 +96,13 @@ def solve_batched(ctx, structures, M=256, kT_ev=0.05, per_batch=256, log=None):
             np.array(status, np.int32))
 
 
-def assemble(ctx, structures, q, offs, status):
+def assemble(ctx, structures, q, offs, status, log=None):
     """Descriptors and targets of the converged structures (status 0)."""
     feats, Zs, qk, seg, qtot = [], [], [], [0], []
+    t0 = time.perf_counter()
     for k, (L, Mc, N, sg, sd) in enumerate(structures):
+        if log and k % 2048 == 0:
+            log(f"descriptors: {k}/{len(structures)} structures, {time.perf_counter() - t0:.1f} s")
         if status[k] != 0:
             continue
         s = System.quartz(L, Mc, N, sigma_ang=sg, seed=sd)
@@ -113,13 +116,16 @@ def assemble(ctx, structures, q, offs, status):
                 seg=np.array(seg, np.int32), qtot=np.array(qtot))
 
 
-def train(ctx, data, steps=3000, lr=2e-3, seed=1, log=None, seconds=None):
+def train(ctx, data, steps=3000, lr=2e-3, seed=1, log=None, seconds=None, init=None):
     """Full-batch Adam on the device; lr drops 4x at half the steps (or half
     of `seconds`).  With `seconds`, training stops at the first 100-step
-    boundary past that time."""
+    boundary past that time.  `init`: parameters to continue from."""
     m = Mlp(ctx, seed=seed)
     m.set_data(data["feat"], data["Z"], data["seg"], data["qtot"], data["q"])
-    m.fit_normalization()
+    if init is None:
+        m.fit_normalization()
+    else:  # continue from trained parameters (their feature normalisation included)
+        m.set_params(init)
     hist = []
     done = 0
     t0 = time.perf_counter()
@@ -162,6 +168,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--train-seconds", type=float, default=None)
+    ap.add_argument("--init", default="", help="continue training from these weights (.npz)")
     ap.add_argument("--batched", action="store_true")
     ap.add_argument("--charges-out", default="")
     ap.add_argument("--charges-in", nargs="*", default=[])
@@ -186,7 +193,7 @@ def main(argv=None):
             offs.extend((offs[-1] + p["offs"][1:]).tolist())
         status = np.concatenate([p["status"] for p in parts])
         iters = np.concatenate([p["iters"] for p in parts])
-        data = assemble(ctx, structures, q, np.array(offs), status)
+        data = assemble(ctx, structures, q, np.array(offs), status, log=log)
         M = int(parts[0]["M"])
     elif a.batched:
         structures = batch_structures(a.first + a.count, a.seed)[a.first:]
@@ -200,7 +207,7 @@ def main(argv=None):
                                   seconds=round(time.perf_counter() - t0, 1))))
             ctx.close()
             return
-        data = assemble(ctx, structures, q, offs, status)
+        data = assemble(ctx, structures, q, offs, status, log=log)
         M = a.M
     else:
         data = generate(ctx, batch_structures(a.count, a.seed), M=a.M, log=log)
@@ -212,7 +219,8 @@ def main(argv=None):
         np.savez_compressed(a.data_out, feat=data["feat"], Z=data["Z"], q=data["q"], seg=data["seg"],
                             qtot=data["qtot"])
     t0 = time.perf_counter()
-    m, hist = train(ctx, data, steps=a.steps, lr=a.lr, log=log, seconds=a.train_seconds)
+    init = np.load(a.init, allow_pickle=False)["params"] if a.init else None
+    m, hist = train(ctx, data, steps=a.steps, lr=a.lr, log=log, seconds=a.train_seconds, init=init)
     ttrain = time.perf_counter() - t0
     q = m.forward()
     err = np.abs(q - data["q"])

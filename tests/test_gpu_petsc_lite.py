@@ -120,7 +120,7 @@ def test_driver_stochastic_mode(tmp_path):
     o = O.System.quartz(2, 2, 2, sigma_ang=0.05, seed=1)
     e = np.linalg.eigvalsh(o.dense_h())
     assert e[191] - 0.02 < out["mu_ha"] < e[192] + 0.02
-    # exact populations at the driver's mu: per-atom moments times the weights
+    # exact populations at ltb_scf's mu: per-atom moments times the weights
     lo, hi = o.gershgorin()
     a, b = (hi - lo) / 2, (hi + lo) / 2
     c = O.fd_coeffs(200, (out["mu_ha"] - b) / a, 0.05 / O.HARTREE_EV / a, 8192)
