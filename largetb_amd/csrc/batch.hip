@@ -797,7 +797,8 @@ int ltb_batch_scc_run(ltb_batch* b, const ltb_scc_params* prm, const double* gue
   int it = 0, n64 = 0, ngraph = 0;
   b->stat_launches = 0;
   b->stat_seconds = b->stat_bytes = 0;
-  hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};
+  GraphExecs<4> graphs;  // by the set of precisions in use (bit 0 FP32, bit 1 FP64)
+  hipGraphExec_t* exec = graphs.e;
   bool warmed[4] = {false, false, false, false};
   int result = LTB_OK;
   for (it = 0; it < prm->max_iter; it++) {
@@ -864,8 +865,6 @@ int ltb_batch_scc_run(ltb_batch* b, const ltb_scc_params* prm, const double* gue
               n32, n64s, S, el);
     }
   }
-  for (hipGraphExec_t e : exec)
-    if (e) (void)hipGraphExecDestroy(e);
   if (p_out) LTB_HIP(c, hipMemcpy(p_out, b->d_pout, sizeof(double) * b->nat, hipMemcpyDeviceToHost));
   int nconv = 0;
   for (int s = 0; s < S; s++) {

@@ -72,6 +72,19 @@ struct ltb_ctx {
 
 int set_err(ltb_ctx* c, int code, const std::string& msg);
 
+// instantiated graphs of one SCC run, destroyed on every exit path
+template <int N>
+struct GraphExecs {
+  hipGraphExec_t e[N] = {};
+  GraphExecs() = default;
+  GraphExecs(const GraphExecs&) = delete;
+  GraphExecs& operator=(const GraphExecs&) = delete;
+  ~GraphExecs() {
+    for (hipGraphExec_t x : e)
+      if (x) (void)hipGraphExecDestroy(x);
+  }
+};
+
 #define LTB_HIP(ctx, call)                                                              \
   do {                                                                                  \
     hipError_t e_ = (call);                                                             \

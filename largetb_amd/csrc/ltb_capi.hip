@@ -800,7 +800,8 @@ int ltb_scc_run(ltb_ctx* c, const ltb_scc_params* prm, const double* guess, doub
   int it = 0, n64 = 0, ngraph = 0;
   DevScalars h{};
   const bool saved_timing = c->timing;
-  hipGraphExec_t exec[2] = {nullptr, nullptr};  // [FP64, FP32]
+  GraphExecs<2> graphs;  // [FP64, FP32]
+  hipGraphExec_t* exec = graphs.e;
   bool warmed[2] = {false, false};
   st = LTB_NOT_CONVERGED;
   for (it = 0; it < prm->max_iter; it++) {
@@ -849,8 +850,6 @@ int ltb_scc_run(ltb_ctx* c, const ltb_scc_params* prm, const double* guess, doub
     if (prec != LTB_FP64 && prm->switch_tol > 0 && h.dq < prm->switch_tol) prec = LTB_FP64;
   }
   c->timing = saved_timing;
-  for (hipGraphExec_t e : exec)
-    if (e) (void)hipGraphExecDestroy(e);
   if (p_out) LTB_HIP(c, hipMemcpy(p_out, c->d_pout, sizeof(double) * c->nat, hipMemcpyDeviceToHost));
   auto t1 = std::chrono::steady_clock::now();
   if (res) {
