@@ -62,6 +62,26 @@ def cpu_baseline(cells, sigma, seed, M, target_s=15.0):
             "scc_iter_s_extrapolated": round(iter_s, 1)}
 
 
+def c0_scc_both(ctx):
+    """The 72-atom 2x2x2 cell (sigma 0.05 A, seed 1, M=200): the whole SCC to
+    1e-6 e by the oracle (FP64, one host thread, getKPE op sequence) and by
+    the device path, with their charge difference."""
+    import numpy as np
+    import largetb_amd as L
+    import oracle as O
+    o = O.System.quartz(2, 2, 2, sigma_ang=0.05, seed=1)
+    t0 = time.perf_counter()
+    r = o.scc(200, tol=1e-6, maxit=300, nthreads=1)
+    cpu_s = time.perf_counter() - t0
+    s = L.System.quartz(2, 2, 2, sigma_ang=0.05, seed=1)
+    ctx.load(s)
+    ctx.scc(200, precision=L.FP64, max_iter=1, tol=1e-30, check=False)  # warm-up
+    g = ctx.scc(200, precision=L.FP64, tol=1e-6, max_iter=300, use_graph=1)
+    return {"cpu_seconds": round(cpu_s, 3), "cpu_iters": r["iters"], "cores": 1,
+            "gpu_seconds": round(g["seconds"], 4), "gpu_iters": g["iters"],
+            "max_dq_e": float(np.abs(g["q"] - r["q"]).max())}
+
+
 def measured_traffic(cells, M, tiling, prec_name):
     """HBM bytes per SpMM launch from the committed rocprofv3 --pmc summary
     (FETCH_SIZE / WRITE_SIZE scaled by a plain-copy calibration), when this
@@ -473,6 +493,7 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cells, args.sigma, args.seed, args.M)
+        out["cpu_baseline"]["c0_full_scc"] = c0_scc_both(ctx)
     ctx.close()
     print(json.dumps(out))
 
