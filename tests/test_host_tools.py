@@ -1,0 +1,40 @@
+"""Host-side tools: the bench contract's multi-GPU refusal, the training-set
+cell generator, and argument checks of the host entry points."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import largetb_amd as L
+from conftest import ROOT
+from largetb_amd.learned import batch_structures
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_training_cells_follow_the_batched_distribution():
+    cells = batch_structures(4096, seed=2026)
+    assert cells == batch_structures(4096, seed=2026)  # reproducible from the seed
+    L_, M_, N_, sig, seeds = (np.array(x) for x in zip(*cells))
+    assert set(L_) == {2, 3, 4, 5} and set(M_) == {2, 3, 4, 5} and set(N_) == {2, 3, 4}
+    assert sig.min() >= 0.02 and sig.max() <= 0.15 and abs(sig.mean() - 0.085) < 0.005
+    assert len(set(seeds.tolist())) == len(cells)
+    atoms = 9 * L_ * M_ * N_
+    assert atoms.min() == 72 and atoms.max() == 900
+
+
+def test_host_entry_points_reject_bad_arguments(tmp_path):
+    with pytest.raises(L.LtbError):
+        L.kpm_dos(np.ones(1), 1.0, 0.0, 10, 4.0)          # fewer than 2 moments
+    with pytest.raises(L.LtbError):
+        L.kpm_dos(np.ones(8), -1.0, 0.0, 10, 4.0)         # a <= 0
+    with pytest.raises(L.LtbError):
+        L.random_vectors(1, 0, 5)
+    with pytest.raises(L.LtbError):
+        L.write_dos(tmp_path / "no" / "dir" / "dos.txt", [0.0], [1.0])
