@@ -64,3 +64,20 @@ def test_adam_training_reduces_loss(ctx):
     p = m.predict(s)
     assert 2 * p.sum() == pytest.approx(s.nelectrons, abs=1e-8)
     m.close()
+
+
+def test_trained_guess_cuts_iterations_and_keeps_the_answer(ctx):
+    """The shipped network on a cell outside its training stream (4x4x3,
+    sigma 0.1 A, seed 777): the SCC from the guess converges to the same
+    answer as from the neutral start (1e-6 e) in far fewer iterations."""
+    from largetb_amd import learned
+    s = L.System.quartz(4, 4, 3, sigma_ang=0.1, seed=777)
+    ctx.load(s)
+    m, info = learned.load(ctx)
+    guess = m.predict(s)
+    m.close()
+    neutral = ctx.scc(256, precision=L.FP64, tol=1e-6, max_iter=300)
+    guided = ctx.scc(256, guess=guess, precision=L.FP64, tol=1e-6, max_iter=300)
+    assert np.abs(2 * guess - neutral["q"]).max() < 0.02
+    assert guided["iters"] < neutral["iters"] - 20
+    assert np.abs(guided["q"] - neutral["q"]).max() < 1e-6
