@@ -7,8 +7,8 @@
 // Layout: atoms of all structures concatenated (BSR with global block
 // columns, block-diagonal across structures).  Structure s keeps its whole
 // exact trace in flight: T_n(s) is norb_s x ld_s (ld_s = norb_s rounded up to
-// its column groups), row-major, at byte offset 8 xoff[s] (FP32 vectors use
-// half the room).
+// its column groups), row-major, at element offset xoff[s] of the ping-pong
+// buffers of its current precision (FP32 and FP64 buffers are separate).
 // A task is (s, column group, 4 apw atoms); a resident grid walks the task
 // slots, skipping structures that are done or in the other precision.
 #include <hip/hip_runtime.h>
@@ -38,63 +38,122 @@ __host__ __device__ inline int tail_width(int norb) {
   return r == 0 ? 0 : (r > 128 ? 256 : (r > 64 ? 128 : 64));
 }
 
-// One task = one column group of one structure x 4*apw consecutive atoms.
+// One task = one column group of one structure x 4*apw consecutive atoms,
+// described by one 64-byte record per grid-stride slot (s = -1: padding), so
+// a workgroup fetches everything it needs about its next task with one
+// scalar load, issued while it still works on the current one.
 // Partials: 2 x 256 doubles per task.
-// active[s]: 0 done, 1 FP32 recursion, 2 FP64 recursion; the FP32 and FP64
-// instances of this kernel each take the structures of their own precision.
-// slots[]: task per grid-stride slot (-1 = padding), laid out so that slot i
-// runs on XCD i % 8 and every task of one column group shares an XCD.
+// act: the structure's state copied in at the start of every SCC iteration
+// (k_batch_mark): 0 done or padding, 1 FP32 recursion, 2 FP64 recursion; the
+// FP32 and FP64 instances of the step kernel each take their own precision.
+// Slot i runs on XCD i % 8 and every task of one column group shares an XCD.
+struct alignas(64) BatchTask {
+  long long xoff;   // element offset of the structure's vectors
+  int s;            // structure, -1 for a padding slot
+  int a0;           // its first atom (global block row)
+  int nat;          // its atoms
+  int ld;           // its row stride (columns of all its groups)
+  int cstart;       // first column of the group
+  int abase;        // first atom of the task, local to the structure
+  int width;        // 256, 128 or 64 columns
+  int row2;         // first of the task's two partial rows
+  int act;          // active[s] of the current SCC iteration
+  int pad[3];
+};
+static_assert(sizeof(BatchTask) == 64, "one scalar load per task");
+
+__global__ __launch_bounds__(256) void k_batch_mark(BatchTask* __restrict__ tasks, int nslots,
+                                                    const int* __restrict__ active) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nslots) {
+    const int s = tasks[i].s;
+    tasks[i].act = s >= 0 ? active[s] : 0;
+  }
+}
+
+// sum the 4 waves' per-column partials in LDS (FP64) and write one row pair;
+// two LDS buffers used in turn, so one barrier per task suffices (a wave
+// writing buffer p again has passed the barrier that every reader of its
+// previous use reached after reading)
+template <typename T, int V>
+__device__ __forceinline__ void flush_task(const T (&sxy)[V], const T (&syy)[V], int wave, int lane,
+                                           double (*sh)[2][256], double* __restrict__ part, size_t row2,
+                                           int pcol) {
+  constexpr int W = 64 * V;
+#pragma unroll
+  for (int v = 0; v < V; v++) {
+    sh[wave][0][lane * V + v] = (double)sxy[v];
+    sh[wave][1][lane * V + v] = (double)syy[v];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 2 * W; e += 256) {
+    const int which = e / W, cc = e - which * W;
+    const double s = (sh[0][which][cc] + sh[1][which][cc]) + (sh[2][which][cc] + sh[3][which][cc]);
+    part[(row2 + which) * 256 + pcol + cc] = s;
+  }
+}
+
 template <typename T, int V, int MODE, bool NT>
 __device__ __forceinline__ void task_pass(const int* __restrict__ rowptr, const int* __restrict__ col,
                                           const T* __restrict__ hv, const T* xc, T* xn, int a0, int ab,
                                           int ae, int ld, int c0, int nvalid, int wave, int lane,
-                                          double* __restrict__ part, size_t row2, int pcol) {
+                                          double (*sh)[2][256], double* __restrict__ part, size_t row2,
+                                          int pcol) {
   T sxy[V], syy[V];
 #pragma unroll
   for (int v = 0; v < V; v++) sxy[v] = syy[v] = T(0);
-  for (int A = ab; A < ae; ++A)
+  for (int A = ab; A < ae; A += 4)
     ltbdev::atom_step<T, V, MODE, NT>(rowptr, col, hv, xc, xn, a0 + A, ld, 0, c0, nvalid, sxy, syy, a0);
-  ltbdev::flush_partials<T, V>(sxy, syy, wave, lane, part, row2, 256, pcol);
+  flush_task<T, V>(sxy, syy, wave, lane, sh, part, row2, pcol);
 }
 
 template <typename T, int MODE, bool NT>
-__global__ __launch_bounds__(256) void k_batch_step(
-    const int* __restrict__ rowptr, const int* __restrict__ col, const T* __restrict__ hv,
-    const char* __restrict__ xcur, char* __restrict__ xnext, int nslots, const int* __restrict__ slots,
-    const int* __restrict__ task_s, const int* __restrict__ task_c0, const int* __restrict__ task_sg,
-    const int* __restrict__ aoff, const int* __restrict__ lds, const size_t* __restrict__ xoffb,
-    const int* __restrict__ active, double* __restrict__ part, int apw) {
+__global__ __launch_bounds__(256) void k_batch_step(const int* __restrict__ rowptr, const int* __restrict__ col,
+                                                    const T* __restrict__ hv, const char* __restrict__ xcur,
+                                                    char* __restrict__ xnext, int nslots,
+                                                    const BatchTask* __restrict__ tasks,
+                                                    double* __restrict__ part, int apw) {
   constexpr int want = sizeof(T) == 4 ? 1 : 2;
+  __shared__ double sh[2][4][2][256];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  for (int i = blockIdx.x; i < nslots; i += gridDim.x) {
-    const int t = slots[i];
-    if (t < 0) continue;
-    const int s = task_s[t];
-    if (active[s] != want) continue;
-    const int cstart = task_c0[t], sg = task_sg[t];
-    const int a0 = aoff[s], nat = aoff[s + 1] - a0, ld = lds[s], nvalid = 4 * nat;
-    const T* xc = reinterpret_cast<const T*>(xcur + xoffb[s]);
-    T* xn = reinterpret_cast<T*>(xnext + xoffb[s]);
-    const int ab = min((sg * 4 + wave) * apw, nat), ae = min(ab + apw, nat);
-    const size_t row2 = (size_t)t * 2;
-    const int width = cstart < (nvalid & ~255) ? 256 : tail_width(nvalid);
-    if (width == 256) {
-      if constexpr (sizeof(T) == 4) {
-        task_pass<T, 4, MODE, NT>(rowptr, col, hv, xc, xn, a0, ab, ae, ld, cstart + 4 * lane, nvalid, wave,
-                                  lane, part, row2, 0);
+  int i = blockIdx.x, buf = 0;
+  if (i >= nslots) return;
+  BatchTask cur = tasks[i];
+  for (; i < nslots; i += gridDim.x) {
+    const BatchTask nxt = tasks[min(i + (int)gridDim.x, nslots - 1)];  // in flight during this task
+    if (cur.act == want) {
+      const T* xc = reinterpret_cast<const T*>(xcur) + cur.xoff;
+      T* xn = reinterpret_cast<T*>(xnext) + cur.xoff;
+      const int nat = cur.nat, nvalid = 4 * nat;
+      // the 4 waves take the task's atoms in turn, so a workgroup works on 4
+      // consecutive atoms at a time
+      const int ab = cur.abase + wave, ae = min(cur.abase + 4 * apw, nat);
+      const size_t row2 = (size_t)cur.row2;
+      if (cur.width == 256) {
+        if constexpr (sizeof(T) == 4) {
+          task_pass<T, 4, MODE, NT>(rowptr, col, hv, xc, xn, cur.a0, ab, ae, cur.ld, cur.cstart + 4 * lane,
+                                    nvalid, wave, lane, sh[buf], part, row2, 0);
+          buf ^= 1;
+        } else {
+          for (int h = 0; h < 2; h++) {
+            task_pass<T, 2, MODE, NT>(rowptr, col, hv, xc, xn, cur.a0, ab, ae, cur.ld,
+                                      cur.cstart + 128 * h + 2 * lane, nvalid, wave, lane, sh[buf], part, row2,
+                                      128 * h);
+            buf ^= 1;
+          }
+        }
+      } else if (cur.width == 128) {
+        task_pass<T, 2, MODE, NT>(rowptr, col, hv, xc, xn, cur.a0, ab, ae, cur.ld, cur.cstart + 2 * lane, nvalid,
+                                  wave, lane, sh[buf], part, row2, 0);
+        buf ^= 1;
       } else {
-        for (int h = 0; h < 2; h++)
-          task_pass<T, 2, MODE, NT>(rowptr, col, hv, xc, xn, a0, ab, ae, ld, cstart + 128 * h + 2 * lane,
-                                    nvalid, wave, lane, part, row2, 128 * h);
+        task_pass<T, 1, MODE, NT>(rowptr, col, hv, xc, xn, cur.a0, ab, ae, cur.ld, cur.cstart + lane, nvalid,
+                                  wave, lane, sh[buf], part, row2, 0);
+        buf ^= 1;
       }
-    } else if (width == 128) {
-      task_pass<T, 2, MODE, NT>(rowptr, col, hv, xc, xn, a0, ab, ae, ld, cstart + 2 * lane, nvalid, wave, lane,
-                                part, row2, 0);
-    } else {
-      task_pass<T, 1, MODE, NT>(rowptr, col, hv, xc, xn, a0, ab, ae, ld, cstart + lane, nvalid, wave, lane,
-                                part, row2, 0);
     }
+    cur = nxt;
   }
 }
 
@@ -411,18 +470,19 @@ struct ltb_batch {
   std::vector<int> aoff, lds;
   std::vector<size_t> xoff;
   std::vector<void*> allocs;
-  int *d_rowptr, *d_col, *d_blkrow, *d_atomst, *d_grow, *d_gcol, *d_aoff, *d_lds, *d_active;
+  int *d_rowptr, *d_col, *d_blkrow, *d_atomst, *d_grow, *d_gcol, *d_aoff, *d_active;
   int nslots = 0;
   ltb_kpe_tiling tiling{256, 4, LTB_SCHED_SWEEP, 8, 8, 1, 2};  // column group, V, -, blocks/CU, atoms/wave, NT, index order
-  int *d_slots = nullptr, *d_task_s = nullptr, *d_task_c0 = nullptr, *d_task_sg = nullptr,
-      *d_tbegin = nullptr, *d_nsg = nullptr;
+  void* d_tasks = nullptr;  // BatchTask per slot
+  int *d_tbegin = nullptr, *d_nsg = nullptr;
   int *d_gc_s, *d_gc_c;
-  size_t* d_xoffb;
   double *d_h0, *d_ht64, *d_diag0, *d_offabs, *d_gval, *d_p0, *d_pin, *d_pout, *d_v, *d_sc;
   float* d_ht32;
   size_t xelems = 0;
-  void *d_xa = nullptr, *d_xb = nullptr;
-  size_t xbytes = 0;
+  // ping-pong vector buffers per precision; FP32 ones packed on their own
+  // (half the address range of FP64 room, which measurably helps the step)
+  void* d_x32[2] = {nullptr, nullptr};
+  void* d_x64[2] = {nullptr, nullptr};
   double *d_part = nullptr, *d_mu1 = nullptr, *d_mom = nullptr, *d_tot = nullptr, *d_W = nullptr,
          *d_X = nullptr, *d_g = nullptr, *d_w = nullptr;
   int M = -1, K = -1;
@@ -475,10 +535,12 @@ double iteration_bytes(const ltb_batch* b, const std::vector<int>& active, int M
 }
 
 void free_tasks(ltb_batch* b) {
-  for (int** p : {&b->d_slots, &b->d_task_s, &b->d_task_c0, &b->d_task_sg, &b->d_tbegin, &b->d_nsg}) {
+  for (int** p : {&b->d_tbegin, &b->d_nsg}) {
     if (*p) (void)hipFree(*p);
     *p = nullptr;
   }
+  if (b->d_tasks) (void)hipFree(b->d_tasks);
+  b->d_tasks = nullptr;
   if (b->d_part) (void)hipFree(b->d_part);
   b->d_part = nullptr;
 }
@@ -495,61 +557,85 @@ int upload_tmp(ltb_batch* b, int** p, const std::vector<int>& h) {
 // then the tail group), each split into blocks of
 // 4*atoms_per_wave atoms.  Column groups go to the least loaded XCD (load =
 // columns x atoms) so that each XCD's L2 serves the gathers of its own
-// groups; slot 8 i + x holds XCD x's i-th task.
+// groups; slot 8 i + x holds XCD x's i-th task.  Task ids (the partial row
+// pairs) run over structures, then column groups, then atom blocks, which is
+// the order k_batch_reduce reads them in.
 int build_tasks(ltb_batch* b) {
   const int apw = b->tiling.atoms_per_wave;
-  std::vector<int> task_s, task_c0, task_sg, tbegin, nsg;
+  std::vector<int> tbegin, nsg;
+  std::vector<BatchTask> all;
   std::vector<std::vector<int>> xcd_tasks(8);
   std::vector<double> xcd_load(8, 0.0);
   for (int s = 0; s < b->S; s++) {
     const int nat = b->aoff[s + 1] - b->aoff[s], norb = 4 * nat, ld = b->lds[s];
     const int ng = (nat + 4 * apw - 1) / (4 * apw), cw = norb & ~255, tw = tail_width(norb);
-    tbegin.push_back((int)task_s.size());
+    tbegin.push_back((int)all.size());
     nsg.push_back(ng);
     for (int c0 = 0; c0 < ld;) {
       const int width = c0 < cw ? 256 : tw;
       const int x = (int)(std::min_element(xcd_load.begin(), xcd_load.end()) - xcd_load.begin());
       xcd_load[x] += (double)width * nat;
       for (int g = 0; g < ng; g++) {
-        xcd_tasks[x].push_back((int)task_s.size());
-        task_s.push_back(s);
-        task_c0.push_back(c0);
-        task_sg.push_back(g);
+        BatchTask t{};
+        t.xoff = (long long)b->xoff[s];
+        t.s = s;
+        t.a0 = b->aoff[s];
+        t.nat = nat;
+        t.ld = ld;
+        t.cstart = c0;
+        t.abase = g * 4 * apw;
+        t.width = width;
+        t.row2 = 2 * (int)all.size();
+        xcd_tasks[x].push_back((int)all.size());
+        all.push_back(t);
       }
       c0 += width;
     }
   }
   size_t per_xcd = 0;
   for (auto& v : xcd_tasks) per_xcd = std::max(per_xcd, v.size());
-  std::vector<int> slots(8 * per_xcd, -1);
+  BatchTask pad{};
+  pad.s = -1;
+  std::vector<BatchTask> slots(8 * per_xcd, pad);
   for (int x = 0; x < 8; x++)
-    for (size_t i = 0; i < xcd_tasks[x].size(); i++) slots[8 * i + x] = xcd_tasks[x][i];
+    for (size_t i = 0; i < xcd_tasks[x].size(); i++) slots[8 * i + x] = all[xcd_tasks[x][i]];
   free_tasks(b);
   b->nslots = (int)slots.size();
-  b->ntasks = (int)task_s.size();
+  b->ntasks = (int)all.size();
   int st;
-  if ((st = upload_tmp(b, &b->d_slots, slots)) || (st = upload_tmp(b, &b->d_task_s, task_s)) ||
-      (st = upload_tmp(b, &b->d_task_c0, task_c0)) || (st = upload_tmp(b, &b->d_task_sg, task_sg)) ||
-      (st = upload_tmp(b, &b->d_tbegin, tbegin)) || (st = upload_tmp(b, &b->d_nsg, nsg)))
-    return st;
+  if ((st = upload_tmp(b, &b->d_tbegin, tbegin)) || (st = upload_tmp(b, &b->d_nsg, nsg))) return st;
+  if (hipMalloc(&b->d_tasks, sizeof(BatchTask) * std::max<size_t>(slots.size(), 1)) != hipSuccess)
+    return set_err(b->ctx, LTB_ERR_NOMEM, "batch task allocation failed");
+  if (!slots.empty() &&
+      hipMemcpy(b->d_tasks, slots.data(), sizeof(BatchTask) * slots.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return set_err(b->ctx, LTB_ERR_DEVICE, "batch task upload failed");
+  // the structures' current states (every SCC iteration copies them in again)
+  hipLaunchKernelGGL(k_batch_mark, dim3((b->nslots + 255) / 256), dim3(256), 0, b->ctx->stream,
+                     (BatchTask*)b->d_tasks, b->nslots, b->d_active);
+  if (hipGetLastError() != hipSuccess) return set_err(b->ctx, LTB_ERR_DEVICE, "batch task marking failed");
   if (hipMalloc((void**)&b->d_part, sizeof(double) * 512 * (size_t)b->ntasks) != hipSuccess)
     return set_err(b->ctx, LTB_ERR_NOMEM, "batch partials allocation failed");
   return LTB_OK;
 }
 
-// ping-pong vector buffers with FP64 room for every structure
+// ping-pong vector buffers of both precisions for every structure (a
+// structure recurses in FP32 or FP64 from one SCC iteration to the next)
 int ensure_vectors(ltb_batch* b) {
   ltb_ctx* c = b->ctx;
-  const size_t xb = b->xelems * sizeof(double);
-  if (b->xbytes >= xb) return LTB_OK;
-  for (void** p : {&b->d_xa, &b->d_xb}) {
-    if (*p) (void)hipFree(*p);
-    *p = nullptr;
+  for (int k = 0; k < 2; k++) {
+    if (!b->d_x32[k]) LTB_HIP(c, hipMalloc(&b->d_x32[k], std::max<size_t>(b->xelems * sizeof(float), 8)));
+    if (!b->d_x64[k]) LTB_HIP(c, hipMalloc(&b->d_x64[k], std::max<size_t>(b->xelems * sizeof(double), 8)));
   }
-  LTB_HIP(c, hipMalloc(&b->d_xa, xb));
-  LTB_HIP(c, hipMalloc(&b->d_xb, xb));
-  b->xbytes = xb;
   return LTB_OK;
+}
+
+// buffers of recursion step n: t_n is read from cur, t_{n+1} written over
+// t_{n-1} in nxt
+template <typename T>
+void step_buffers(const ltb_batch* b, int n, void** cur, void** nxt) {
+  void* const* x = sizeof(T) == 4 ? b->d_x32 : b->d_x64;
+  *cur = x[n & 1];
+  *nxt = n == 0 ? x[1] : x[(n & 1) ^ 1];
 }
 
 // resident grid: a multiple of 8 blocks keeps slot i on XCD i % 8 through the
@@ -565,8 +651,7 @@ void launch_step(ltb_batch* b, int grid, const void* cur, void* nxt) {
   const T* hv = sizeof(T) == 4 ? (const T*)b->d_ht32 : (const T*)b->d_ht64;
   auto kern = b->tiling.nontemporal ? k_batch_step<T, MODE, true> : k_batch_step<T, MODE, false>;
   hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, b->ctx->stream, b->d_rowptr, b->d_col, hv,
-                     (const char*)cur, (char*)nxt, b->nslots, b->d_slots, b->d_task_s, b->d_task_c0,
-                     b->d_task_sg, b->d_aoff, b->d_lds, b->d_xoffb, b->d_active, b->d_part,
+                     (const char*)cur, (char*)nxt, b->nslots, (const BatchTask*)b->d_tasks, b->d_part,
                      b->tiling.atoms_per_wave);
 }
 
@@ -596,11 +681,16 @@ int batch_moments(ltb_batch* b, int M, int precs, bool capturing) {
   }
   for (int n = 0; n < nsteps; n++) {
     const int mode = n == 0 ? 0 : (n == 1 ? 1 : 2);
-    void* cur = (n & 1) ? b->d_xb : b->d_xa;
-    void* nxt = (n == 0) ? b->d_xb : ((n & 1) ? b->d_xa : b->d_xb);
+    void *cur, *nxt;
     if (timed) LTB_HIP(c, hipEventRecord(b->events[2 * n], c->stream));
-    if (precs & 1) launch_step<float>(b, mode, grid, cur, nxt);
-    if (precs & 2) launch_step<double>(b, mode, grid, cur, nxt);
+    if (precs & 1) {
+      step_buffers<float>(b, n, &cur, &nxt);
+      launch_step<float>(b, mode, grid, cur, nxt);
+    }
+    if (precs & 2) {
+      step_buffers<double>(b, n, &cur, &nxt);
+      launch_step<double>(b, mode, grid, cur, nxt);
+    }
     if (timed) LTB_HIP(c, hipEventRecord(b->events[2 * n + 1], c->stream));
     hipLaunchKernelGGL(k_batch_reduce, dim3((b->ncols + 255) / 256), dim3(256), 0, c->stream, b->d_part,
                        b->ncols, b->d_gc_s, b->d_gc_c, b->d_tbegin, b->d_nsg, b->d_aoff, b->d_active, n,
@@ -622,6 +712,8 @@ int batch_iteration(ltb_batch* b, const ltb_scc_params* prm, int precs, bool cap
   hipLaunchKernelGGL(k_batch_rescale, dim3((b->nnzb * 16 + 255) / 256), dim3(256), 0, c->stream, b->d_h0,
                      b->d_blkrow, b->d_col, b->d_atomst, b->d_v, b->d_sc, b->d_active, b->nnzb, b->d_ht64,
                      b->d_ht32);
+  hipLaunchKernelGGL(k_batch_mark, dim3((b->nslots + 255) / 256), dim3(256), 0, c->stream,
+                     (BatchTask*)b->d_tasks, b->nslots, b->d_active);
   int st = batch_moments(b, M, precs, capturing);
   if (st) return st;
   hipLaunchKernelGGL(k_batch_weights, dim3((K + 255) / 256, b->S), dim3(256), 0, c->stream, b->d_tot,
@@ -702,8 +794,6 @@ int ltb_batch_create(ltb_ctx* c, int nstruct, const ltb_system* const* systems, 
     }
   }
   b->xelems = b->xoff.back();
-  std::vector<size_t> xoffb(b->xoff.begin(), b->xoff.end() - 1);
-  for (size_t& o : xoffb) o *= sizeof(double);  // room for FP64 vectors; FP32 uses half
   b->ncols = (int)gc_s.size();
   std::vector<int> ones(nstruct, 1);
   std::vector<double> sc((size_t)nstruct * NSC, 0.0);
@@ -712,9 +802,9 @@ int ltb_batch_create(ltb_ctx* c, int nstruct, const ltb_system* const* systems, 
   if ((st = bupload(b, &b->d_rowptr, rowptr)) || (st = bupload(b, &b->d_col, col)) ||
       (st = bupload(b, &b->d_blkrow, blkrow)) || (st = bupload(b, &b->d_atomst, atomst)) ||
       (st = bupload(b, &b->d_grow, grow)) || (st = bupload(b, &b->d_gcol, gcol)) ||
-      (st = bupload(b, &b->d_aoff, b->aoff)) || (st = bupload(b, &b->d_lds, b->lds)) ||
+      (st = bupload(b, &b->d_aoff, b->aoff)) ||
       (st = bupload(b, &b->d_active, ones)) || (st = bupload(b, &b->d_gc_s, gc_s)) || (st = bupload(b, &b->d_gc_c, gc_c)) ||
-      (st = bupload(b, &b->d_xoffb, xoffb)) || (st = bupload(b, &b->d_h0, h0)) ||
+      (st = bupload(b, &b->d_h0, h0)) ||
       (st = bupload(b, &b->d_diag0, diag0)) || (st = bupload(b, &b->d_offabs, offabs)) ||
       (st = bupload(b, &b->d_gval, gval)) || (st = bupload(b, &b->d_p0, p0)) ||
       (st = bupload(b, &b->d_pin, p0)) || (st = bupload(b, &b->d_pout, p0)) ||
@@ -751,7 +841,7 @@ void ltb_batch_destroy(ltb_batch* b) {
   for (void* p : b->allocs) (void)hipFree(p);
   free_tasks(b);
   for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
-  for (void* p : {(void*)b->d_xa, (void*)b->d_xb, (void*)b->d_mom, (void*)b->d_tot, (void*)b->d_W,
+  for (void* p : {b->d_x32[0], b->d_x32[1], b->d_x64[0], b->d_x64[1], (void*)b->d_mom, (void*)b->d_tot, (void*)b->d_W,
                   (void*)b->d_X, (void*)b->d_g, (void*)b->d_w})
     if (p) (void)hipFree(p);
   delete b;
@@ -949,12 +1039,14 @@ int ltb_batch_autotune(ltb_batch* b, int precision, int trial_steps, ltb_kpe_til
         const int grid = step_grid(b);
         auto step = [&](int n) {
           const int mode = n == 0 ? 0 : (n == 1 ? 1 : 2);
-          void* cur = (n & 1) ? b->d_xb : b->d_xa;
-          void* nxt = (n == 0) ? b->d_xb : ((n & 1) ? b->d_xa : b->d_xb);
-          if (precision == LTB_FP64)
+          void *cur, *nxt;
+          if (precision == LTB_FP64) {
+            step_buffers<double>(b, n, &cur, &nxt);
             launch_step<double>(b, mode, grid, cur, nxt);
-          else
+          } else {
+            step_buffers<float>(b, n, &cur, &nxt);
             launch_step<float>(b, mode, grid, cur, nxt);
+          }
         };
         step(0);
         step(1);
