@@ -30,7 +30,9 @@
 //    neighbour re-reads (a few hundred to ~1300 atoms away) are still in that
 //    XCD's L2.
 // Per-column dot partials are summed in LDS per workgroup, written once, and
-// folded into per-atom moments by k_cheb_reduce.
+// folded into per-atom moments (fold_columns): under the sweep schedule by the
+// first workgroups of the next step's launch, otherwise (and after a chunk's
+// last step) by k_cheb_reduce.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -43,6 +45,70 @@ namespace {
 
 using ltbdev::atom_step;
 using ltbdev::flush_partials;
+
+// Fold the workgroup partials of step n into per-atom moments:
+//   n == 0: mu_1 = <t_1, t_0>,           mu_2 = 2 <t_1, t_1> - 1
+//   n >= 1: mu_{2n+1} = 2 <t_{n+1}, t_n> - mu_1,  mu_{2n+2} = 2 |t_{n+1}|^2 - 1
+// and sum the 4 columns (orbitals) of each atom.  mom is [M+1][nat].  The
+// partial rows holding column c are rows [base(c), base(c) + count): blocked
+// schedule base 0, count n_sg; sweep schedule base (c/W % 8)*slots, count slots.
+struct FoldArgs {
+  const double* part;  // partial rows of the step being folded (nullptr: none)
+  int count, sweep_W, slots, ld, nvalid, j0, n, M;
+  double* mu1col;
+  double* mom;
+  int nat;
+  double* momorb;  // per-orbital moments [M+1][norb], or nullptr
+  int norb;
+};
+
+// the 256 threads of a workgroup fold columns 64 cb .. 64 cb + 63
+__device__ __forceinline__ void fold_columns(const FoldArgs& f, int cb) {
+  const int lane = threadIdx.x & 63, slice = threadIdx.x >> 6;
+  const int c = cb * 64 + lane;
+  const int base = f.sweep_W > 0 ? ((c / f.sweep_W) & 7) * f.slots : 0;
+  double sxy = 0, syy = 0;
+  for (int s = slice; s < f.count; s += 4) {
+    sxy += f.part[(size_t)(2 * (base + s)) * f.ld + c];
+    syy += f.part[(size_t)(2 * (base + s) + 1) * f.ld + c];
+  }
+  __shared__ double sh[2][4][64];
+  sh[0][slice][lane] = sxy;
+  sh[1][slice][lane] = syy;
+  __syncthreads();
+  if (slice == 0) {
+    sxy = (sh[0][0][lane] + sh[0][1][lane]) + (sh[0][2][lane] + sh[0][3][lane]);
+    syy = (sh[1][0][lane] + sh[1][1][lane]) + (sh[1][2][lane] + sh[1][3][lane]);
+    const bool valid = c < f.nvalid;
+    const int n = f.n, M = f.M;
+    double modd;
+    if (n == 0) {
+      modd = sxy;
+      f.mu1col[c] = sxy;
+    } else {
+      modd = 2.0 * sxy - f.mu1col[c];
+    }
+    double meven = 2.0 * syy - (valid ? 1.0 : 0.0);
+    if (f.momorb && valid) {  // per-orbital moments [M+1][norb] (diag(P) per orbital)
+      const int j = f.j0 + c;
+      if (n == 0) f.momorb[j] = 1.0;
+      if (2 * n + 1 <= M) f.momorb[(size_t)(2 * n + 1) * f.norb + j] = modd;
+      if (2 * n + 2 <= M) f.momorb[(size_t)(2 * n + 2) * f.norb + j] = meven;
+    }
+    modd += __shfl_xor(modd, 1);
+    modd += __shfl_xor(modd, 2);
+    meven += __shfl_xor(meven, 1);
+    meven += __shfl_xor(meven, 2);
+    if ((lane & 3) == 0 && valid) {
+      const int atom = (f.j0 + c) >> 2;
+      if (n == 0) f.mom[atom] = 4.0;
+      const int mo = 2 * n + 1, me = 2 * n + 2;
+      if (mo <= M) f.mom[(size_t)mo * f.nat + atom] = modd;
+      if (me <= M) f.mom[(size_t)me * f.nat + atom] = meven;
+    }
+  }
+  __syncthreads();  // sh is free again
+}
 
 template <typename T, int V, int MODE, bool NT>
 __global__ __launch_bounds__(256) void k_cheb_blocked(const int* __restrict__ rowptr,
@@ -79,8 +145,12 @@ __global__ __launch_bounds__(256) void k_cheb_sweep(const int* __restrict__ rowp
                                                     T* __restrict__ xnext, int nat, int ld, int j0,
                                                     int nvalid, int n_cg, int slots,
                                                     const int* __restrict__ visit,
-                                                    double* __restrict__ part) {
+                                                    double* __restrict__ part, FoldArgs prev) {
   constexpr int W = 64 * V;
+  // the previous step's partials (complete: it was the previous launch) are
+  // folded here instead of in a launch of their own
+  if (prev.part)
+    for (int cb = blockIdx.x; cb < prev.ld / 64; cb += gridDim.x) fold_columns(prev, cb);
   const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -98,60 +168,9 @@ __global__ __launch_bounds__(256) void k_cheb_sweep(const int* __restrict__ rowp
   }
 }
 
-// Fold the workgroup partials of step n into per-atom moments:
-//   n == 0: mu_1 = <t_1, t_0>,           mu_2 = 2 <t_1, t_1> - 1
-//   n >= 1: mu_{2n+1} = 2 <t_{n+1}, t_n> - mu_1,  mu_{2n+2} = 2 |t_{n+1}|^2 - 1
-// and sum the 4 columns (orbitals) of each atom.  mom is [M+1][nat].  The
-// partial rows holding column c are rows [base(c), base(c) + count): blocked
-// schedule base 0, count n_sg; sweep schedule base (c/W % 8)*slots, count slots.
-__global__ __launch_bounds__(256) void k_cheb_reduce(const double* __restrict__ part, int count,
-                                                     int sweep_W, int slots, int ld, int nvalid,
-                                                     int j0, int n, int M,
-                                                     double* __restrict__ mu1col,
-                                                     double* __restrict__ mom, int nat,
-                                                     double* __restrict__ momorb, int norb) {
-  const int lane = threadIdx.x & 63, slice = threadIdx.x >> 6;
-  const int c = blockIdx.x * 64 + lane;
-  const int base = sweep_W > 0 ? ((c / sweep_W) & 7) * slots : 0;
-  double sxy = 0, syy = 0;
-  for (int s = slice; s < count; s += 4) {
-    sxy += part[(size_t)(2 * (base + s)) * ld + c];
-    syy += part[(size_t)(2 * (base + s) + 1) * ld + c];
-  }
-  __shared__ double sh[2][4][64];
-  sh[0][slice][lane] = sxy;
-  sh[1][slice][lane] = syy;
-  __syncthreads();
-  if (slice != 0) return;
-  sxy = (sh[0][0][lane] + sh[0][1][lane]) + (sh[0][2][lane] + sh[0][3][lane]);
-  syy = (sh[1][0][lane] + sh[1][1][lane]) + (sh[1][2][lane] + sh[1][3][lane]);
-  const bool valid = c < nvalid;
-  double modd;
-  if (n == 0) {
-    modd = sxy;
-    mu1col[c] = sxy;
-  } else {
-    modd = 2.0 * sxy - mu1col[c];
-  }
-  double meven = 2.0 * syy - (valid ? 1.0 : 0.0);
-  if (momorb && valid) {  // per-orbital moments [M+1][norb] (diag(P) per orbital)
-    const int j = j0 + c;
-    if (n == 0) momorb[j] = 1.0;
-    if (2 * n + 1 <= M) momorb[(size_t)(2 * n + 1) * norb + j] = modd;
-    if (2 * n + 2 <= M) momorb[(size_t)(2 * n + 2) * norb + j] = meven;
-  }
-  modd += __shfl_xor(modd, 1);
-  modd += __shfl_xor(modd, 2);
-  meven += __shfl_xor(meven, 1);
-  meven += __shfl_xor(meven, 2);
-  if ((lane & 3) == 0 && valid) {
-    const int atom = (j0 + c) >> 2;
-    if (n == 0) mom[atom] = 4.0;
-    const int mo = 2 * n + 1, me = 2 * n + 2;
-    if (mo <= M) mom[(size_t)mo * nat + atom] = modd;
-    if (me <= M) mom[(size_t)me * nat + atom] = meven;
-  }
-}
+// the partials of the last step of a chunk (and of every step under the
+// blocked schedule)
+__global__ __launch_bounds__(256) void k_cheb_reduce(FoldArgs f) { fold_columns(f, blockIdx.x); }
 
 __global__ __launch_bounds__(256) void k_mom_total(const double* __restrict__ mom, int nat,
                                                    double* __restrict__ tot) {
@@ -248,52 +267,54 @@ Plan make_plan(const ltb_ctx* c) {
   return p;
 }
 
+// part: where this step's partials go; prev: the previous step's partials to
+// fold first (sweep schedule only; prev.part == nullptr for none)
 template <typename T, int V, int MODE, bool NT>
 void launch_one(const ltb_ctx* c, const T* hv, const T* xcur, T* xnext, int ldc, int j0, int nvalid,
-                const Plan& p) {
+                const Plan& p, double* part, const FoldArgs& prev) {
   const int n_cg = ldc / (64 * V);
   if (p.schedule == LTB_SCHED_SWEEP) {
     hipLaunchKernelGGL((k_cheb_sweep<T, V, MODE, NT>), dim3(8 * p.slots), dim3(256), 0, c->stream,
                        c->d_rowptr, c->d_col, hv, xcur, xnext, c->nat, ldc, j0, nvalid, n_cg,
-                       p.slots, p.order == 1 ? c->d_visit : nullptr, c->d_part);
+                       p.slots, p.order == 1 ? c->d_visit : nullptr, part, prev);
   } else {
     hipLaunchKernelGGL((k_cheb_blocked<T, V, MODE, NT>), dim3(n_cg * p.n_sg), dim3(256), 0,
                        c->stream, c->d_rowptr, c->d_col, hv, xcur, xnext, c->nat, ldc, j0, nvalid,
-                       p.apw, p.n_sg, c->d_part);
+                       p.apw, p.n_sg, part);
   }
 }
 
 template <typename T, int V, int MODE>
 void launch_nt(const ltb_ctx* c, const T* hv, const T* xcur, T* xnext, int ldc, int j0, int nvalid,
-               const Plan& p) {
+               const Plan& p, double* part, const FoldArgs& prev) {
   if (p.nt)
-    launch_one<T, V, MODE, true>(c, hv, xcur, xnext, ldc, j0, nvalid, p);
+    launch_one<T, V, MODE, true>(c, hv, xcur, xnext, ldc, j0, nvalid, p, part, prev);
   else
-    launch_one<T, V, MODE, false>(c, hv, xcur, xnext, ldc, j0, nvalid, p);
+    launch_one<T, V, MODE, false>(c, hv, xcur, xnext, ldc, j0, nvalid, p, part, prev);
 }
 
 template <typename T, int V>
 void launch_mode(const ltb_ctx* c, int mode, const T* hv, const T* xcur, T* xnext, int ldc, int j0,
-                 int nvalid, const Plan& p) {
+                 int nvalid, const Plan& p, double* part, const FoldArgs& prev) {
   if (mode == 0)
-    launch_nt<T, V, 0>(c, hv, xcur, xnext, ldc, j0, nvalid, p);
+    launch_nt<T, V, 0>(c, hv, xcur, xnext, ldc, j0, nvalid, p, part, prev);
   else if (mode == 1)
-    launch_nt<T, V, 1>(c, hv, xcur, xnext, ldc, j0, nvalid, p);
+    launch_nt<T, V, 1>(c, hv, xcur, xnext, ldc, j0, nvalid, p, part, prev);
   else if (mode == 2)
-    launch_nt<T, V, 2>(c, hv, xcur, xnext, ldc, j0, nvalid, p);
+    launch_nt<T, V, 2>(c, hv, xcur, xnext, ldc, j0, nvalid, p, part, prev);
   else
-    launch_nt<T, V, 3>(c, hv, xcur, xnext, ldc, j0, nvalid, p);
+    launch_nt<T, V, 3>(c, hv, xcur, xnext, ldc, j0, nvalid, p, part, prev);
 }
 
 template <typename T>
 void launch_step(const ltb_ctx* c, int mode, const T* hv, const T* xcur, T* xnext, int ldc, int j0,
-                 int nvalid, const Plan& p) {
+                 int nvalid, const Plan& p, double* part, const FoldArgs& prev = FoldArgs{}) {
   if (p.V == 4)
-    launch_mode<T, 4>(c, mode, hv, xcur, xnext, ldc, j0, nvalid, p);
+    launch_mode<T, 4>(c, mode, hv, xcur, xnext, ldc, j0, nvalid, p, part, prev);
   else if (p.V == 2)
-    launch_mode<T, 2>(c, mode, hv, xcur, xnext, ldc, j0, nvalid, p);
+    launch_mode<T, 2>(c, mode, hv, xcur, xnext, ldc, j0, nvalid, p, part, prev);
   else
-    launch_mode<T, 1>(c, mode, hv, xcur, xnext, ldc, j0, nvalid, p);
+    launch_mode<T, 1>(c, mode, hv, xcur, xnext, ldc, j0, nvalid, p, part, prev);
 }
 
 template <typename T>
@@ -302,7 +323,8 @@ int ensure_buffers(ltb_ctx* c, const Plan& p, int M) {
   int st;
   if ((st = ensure_buffer(c, &c->d_xa, &c->xabytes, xb))) return st;
   if ((st = ensure_buffer(c, &c->d_xb, &c->xbbytes, xb))) return st;
-  if ((st = ensure_buffer(c, (void**)&c->d_part, &c->partbytes, p.part_rows * 2 * p.ld * sizeof(double))))
+  // two sets of partial rows: step n writes set n % 2 while it folds set (n - 1) % 2
+  if ((st = ensure_buffer(c, (void**)&c->d_part, &c->partbytes, 2 * p.part_rows * 2 * p.ld * sizeof(double))))
     return st;
   if ((st = ensure_buffer(c, (void**)&c->d_mu1, &c->mu1bytes, (size_t)p.ld * sizeof(double)))) return st;
   if (M >= 0) {
@@ -356,22 +378,29 @@ int run_moments(ltb_ctx* c, int M) {
       }
     }
     const bool sweep = pc.schedule == LTB_SCHED_SWEEP;
+    const size_t half = p.part_rows * 2 * (size_t)p.ld;
+    auto fold_args = [&](int n) {
+      return FoldArgs{c->d_part + (n & 1) * half, sweep ? pc.slots : pc.n_sg, sweep ? pc.W : 0, pc.slots,
+                      ldc, nvalid, j0, n, M, c->d_mu1, c->d_mom, c->nat,
+                      c->keep_orbital ? c->d_momorb : nullptr, c->norb};
+    };
     for (int n = 0; n < nsteps; n++) {
       const int mode = n == 0 ? 0 : (n == 1 ? 1 : 2);
       T* cur = (n & 1) ? B : A;
       T* nxt = (n == 0) ? B : ((n & 1) ? A : B);
+      // under the sweep schedule step n folds step n - 1's partials itself
+      const FoldArgs prev = sweep && n > 0 ? fold_args(n - 1) : FoldArgs{};
       if (c->timing) LTB_HIP(c, hipEventRecord(c->events[2 * launches], c->stream));
-      launch_step<T>(c, mode, hv, cur, nxt, ldc, j0, nvalid, pc);
+      launch_step<T>(c, mode, hv, cur, nxt, ldc, j0, nvalid, pc, c->d_part + (n & 1) * half, prev);
       if (c->timing) LTB_HIP(c, hipEventRecord(c->events[2 * launches + 1], c->stream));
       LTB_HIP(c, hipGetLastError());
       const int streams = mode == 0 ? 1 : (mode == 1 ? 2 : 3);
       bytes += (double)streams * sizeof(T) * c->norb * (double)nvalid + hbytes;
       launches++;
-      hipLaunchKernelGGL(k_cheb_reduce, dim3(ldc / 64), dim3(256), 0, c->stream, c->d_part,
-                         sweep ? pc.slots : pc.n_sg, sweep ? pc.W : 0, pc.slots, ldc, nvalid, j0, n,
-                         M, c->d_mu1, c->d_mom, c->nat, c->keep_orbital ? c->d_momorb : nullptr,
-                         c->norb);
-      LTB_HIP(c, hipGetLastError());
+      if (!sweep || n == nsteps - 1) {
+        hipLaunchKernelGGL(k_cheb_reduce, dim3(ldc / 64), dim3(256), 0, c->stream, fold_args(n));
+        LTB_HIP(c, hipGetLastError());
+      }
     }
   }
   hipLaunchKernelGGL(k_mom_total, dim3(M + 1), dim3(256), 0, c->stream, c->d_mom, c->nat,
@@ -408,11 +437,11 @@ int time_steps(ltb_ctx* c, int steps, double* seconds, double* bytes) {
   hipEvent_t e0, e1;
   LTB_HIP(c, hipEventCreate(&e0));
   LTB_HIP(c, hipEventCreate(&e1));
-  launch_step<T>(c, 2, hv, (T*)c->d_xa, (T*)c->d_xb, ldc, 0, nvalid, p);  // warm
+  launch_step<T>(c, 2, hv, (T*)c->d_xa, (T*)c->d_xb, ldc, 0, nvalid, p, c->d_part);  // warm
   LTB_HIP(c, hipEventRecord(e0, c->stream));
   for (int i = 0; i < steps; i++)
     launch_step<T>(c, 2, hv, (i & 1) ? (T*)c->d_xb : (T*)c->d_xa, (i & 1) ? (T*)c->d_xa : (T*)c->d_xb,
-                   ldc, 0, nvalid, p);
+                   ldc, 0, nvalid, p, c->d_part);
   LTB_HIP(c, hipEventRecord(e1, c->stream));
   LTB_HIP(c, hipEventSynchronize(e1));
   LTB_HIP(c, hipGetLastError());
@@ -457,7 +486,7 @@ int run_block_moments(ltb_ctx* c, int M, int nvec, const double* d_r2, double* d
     const int mode = n == 0 ? 3 : 2;
     T* cur = (n & 1) ? B : A;
     T* nxt = (n & 1) ? A : B;
-    launch_step<T>(c, mode, hv, cur, nxt, ld, 0, nvec, p);
+    launch_step<T>(c, mode, hv, cur, nxt, ld, 0, nvec, p, c->d_part);
     LTB_HIP(c, hipGetLastError());
     hipLaunchKernelGGL(k_block_reduce, dim3(ld / 64), dim3(256), 0, c->stream, c->d_part,
                        sweep ? p.slots : p.n_sg, sweep ? p.W : 0, p.slots, ld, nvec, n, M, c->d_mu1, d_r2,
