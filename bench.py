@@ -59,7 +59,19 @@ def cpu_baseline(cells, sigma, seed, M, target_s=15.0):
     return {"value": round(gbs, 3), "unit": "GB/s", "cores": cores, "kind": "port",
             "sample": f"{ncol} of {n} unit-vector columns, M={M}, FP64 CSR SpMV + BLAS-1 per "
                       f"step (getKPE op sequence), {t:.1f} s",
-            "scc_iter_s_extrapolated": round(iter_s, 1)}
+            "scc_iter_s_extrapolated": round(iter_s, 1), "nproc": os.cpu_count(), "cpu_model": cpu_model()}
+
+
+def cpu_model():
+    """The host CPU's model name (first processor in /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def c0_scc_both(ctx):
@@ -477,6 +489,8 @@ def main():
         "metric": METRIC, "value": round(value, 2), "unit": "GB/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(wall / args.steps * 1e3, 3), "higher_is_better": True,
+        # diagonal moments [T_m]_jj, m = 0..M, of every unit vector j, per second of SCC iteration
+        "moments_per_s": round((args.M + 1) * s.norb * args.steps / wall, 1),
         "scaling": "weak", "vs_baseline": None, "dtype": "fp32" if prec == L.FP32_ACC64 else "f64",
         "data": "synthetic alpha-quartz, Gaussian displacements from the seed",
         "config": {"workload": f"SCC iteration, {s.nat}-atom alpha-quartz "
