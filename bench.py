@@ -144,6 +144,7 @@ def time_to_solution(args):
     ctx.set_populations(None)
     lo, hi = ctx.gershgorin()
     ctx.rescale((hi - lo) / 2, (hi + lo) / 2)
+    t64, _ = ctx.autotune(L.FP64, 3)  # each precision keeps its own tiling
     t32, _ = ctx.autotune(L.FP32_ACC64, 3)
     ctx.scc(args.M, precision=L.FP32_ACC64, max_iter=1, tol=1e-30, check=False)
     ctx.scc(args.M, precision=L.FP64, max_iter=1, tol=1e-30, check=False)
@@ -172,7 +173,7 @@ def time_to_solution(args):
                                      eband_fp=r["eband_fp"], mu=r["mu_ha"])
     ref = runs["plain_fp64"]
     out = {"metric": METRIC, "workload": f"{s.nat}-atom alpha-quartz {'x'.join(map(str, args.cells))}",
-           "M": args.M, "sigma_ang": args.sigma, "seed": args.seed, "tiling_fp32": t32}
+           "M": args.M, "sigma_ang": args.sigma, "seed": args.seed, "tiling_fp32": t32, "tiling_fp64": t64}
     for k, v in runs.items():
         d = {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items() if kk != "q"}
         d["max_dq_vs_plain_e"] = float(np.abs(v["q"] - ref["q"]).max())
@@ -304,8 +305,9 @@ def scc_segment(args):
     ctx.set_populations(None)
     lo, hi = ctx.gershgorin()
     ctx.rescale((hi - lo) / 2, (hi + lo) / 2)
-    tiling, gbs = ctx.autotune(prec, 3)
-    log(f"autotune {tiling} {gbs:.0f} GB/s")
+    for p in ([L.FP64, L.FP32_ACC64] if mixed else [L.FP64]):  # each precision keeps its own tiling
+        tiling, gbs = ctx.autotune(p, 3)
+        log(f"autotune {'fp64' if p == L.FP64 else 'fp32'}: {tiling} {gbs:.0f} GB/s")
     ctx.scc(args.M, precision=prec, max_iter=1, tol=1e-30, check=False)  # warm-up, not counted
     extra = 0.0
     if st is not None:

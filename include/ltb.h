@@ -329,12 +329,15 @@ typedef struct {
   int order;          /* sweep visiting order of the atoms: 0 auto (= 2), 1 spatial
                          strips (bonded atoms close in sequence), 2 atom index   */
 } ltb_kpe_tiling;
+/* The context keeps one tiling per recursion precision (FP32, FP64); this
+ * sets both. */
 int ltb_set_kpe_tiling(ltb_ctx* ctx, const ltb_kpe_tiling* t);
 /* the tiling the kernel would use for the loaded system (auto fields resolved) */
 int ltb_get_kpe_tiling(ltb_ctx* ctx, int precision, ltb_kpe_tiling* out);
 /* Time `trial_steps` recursion steps of every candidate tiling on the loaded
- * system, keep the fastest (set on the context) and report it with its
- * algorithmic GB/s. */
+ * system, keep the fastest as the tiling of `precision` (the other
+ * precision's tiling is left as it was) and report it with its algorithmic
+ * GB/s. */
 int ltb_kpe_autotune(ltb_ctx* ctx, int precision, int trial_steps, ltb_kpe_tiling* best,
                      double* best_gbs);
 

@@ -233,7 +233,7 @@ struct Plan {
 
 template <typename T>
 Plan make_plan(const ltb_ctx* c) {
-  const KpeTiling& t = c->tiling;
+  const KpeTiling& t = c->tiling[sizeof(T) == 8 ? 1 : 0];
   Plan p{};
   const bool big = c->norb >= 4096;
   p.V = t.cols_per_lane > 0 ? t.cols_per_lane : (big ? (sizeof(T) == 4 ? 2 : 1) : 1);

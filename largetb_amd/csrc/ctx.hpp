@@ -58,8 +58,9 @@ struct ltb_ctx {
   // Fermi / density
   double *d_W = nullptr, *d_X = nullptr, *d_g = nullptr, *d_c = nullptr, *d_w = nullptr;
   int fd_K = 0, fd_M = 0;
-  // tiling of the Chebyshev kernel (0 / -1 fields = auto)
-  ltb_kpe_tiling tiling = {0, 0, 0, 0, 0, -1, 0};
+  // tiling of the Chebyshev kernel per recursion precision, [0] FP32 and
+  // [1] FP64 (0 / -1 fields = auto)
+  ltb_kpe_tiling tiling[2] = {{0, 0, 0, 0, 0, -1, 0}, {0, 0, 0, 0, 0, -1, 0}};
   // statistics of the last ltb_kpe_moments
   int stat_launches = 0;
   double stat_seconds = 0, stat_bytes = 0;

@@ -937,7 +937,7 @@ int ltb_set_kpe_tiling(ltb_ctx* c, const ltb_kpe_tiling* t) {
       t->schedule > 2 || t->order < 0 || t->order > 2 ||
       (t->cols_per_lane != 0 && t->cols_per_lane != 1 && t->cols_per_lane != 2 && t->cols_per_lane != 4))
     return set_err(c, LTB_ERR_ARG, "bad tiling");
-  c->tiling = *t;
+  c->tiling[0] = c->tiling[1] = *t;
   return LTB_OK;
 }
 
@@ -953,7 +953,8 @@ int ltb_kpe_autotune(ltb_ctx* c, int precision, int trial_steps, ltb_kpe_tiling*
   int st = need_system(c);
   if (st) return st;
   if (trial_steps < 1) trial_steps = 3;
-  const ltb_kpe_tiling saved = c->tiling;
+  ltb_kpe_tiling& cur = c->tiling[precision == LTB_FP64 ? 1 : 0];  // only this precision's tiling is tuned
+  const ltb_kpe_tiling saved = cur;
   std::vector<ltb_kpe_tiling> cand;
   const int chunks[] = {2048, 4096, 8192};
   const int vs[] = {1, 2, 4};
@@ -966,7 +967,7 @@ int ltb_kpe_autotune(ltb_ctx* c, int precision, int trial_steps, ltb_kpe_tiling*
   double best_rate = -1;
   ltb_kpe_tiling win = saved;
   for (const ltb_kpe_tiling& t : cand) {
-    c->tiling = t;
+    cur = t;
     ltb_kpe_tiling r;
     kpe_resolved_tiling(c, precision, &r);
     if (r.schedule != t.schedule) continue;  // sweep needs >= 8 column groups
@@ -974,7 +975,7 @@ int ltb_kpe_autotune(ltb_ctx* c, int precision, int trial_steps, ltb_kpe_tiling*
     double secs = 0, bytes = 0;
     st = kpe_time_steps(c, precision, trial_steps, &secs, &bytes);
     if (st) {
-      c->tiling = saved;
+      cur = saved;
       return st;
     }
     double rate = bytes / secs;
@@ -983,7 +984,7 @@ int ltb_kpe_autotune(ltb_ctx* c, int precision, int trial_steps, ltb_kpe_tiling*
       win = t;
     }
   }
-  c->tiling = win;
+  cur = win;
   if (best) kpe_resolved_tiling(c, precision, best);
   if (best_gbs) *best_gbs = best_rate / 1e9;
   return LTB_OK;

@@ -117,9 +117,12 @@ def test_autotune_picks_a_valid_tiling(ctx):
     s = L.System.quartz(8, 8, 8, sigma_ang=0.05, seed=1)
     prepare(ctx, s)
     try:
+        best64, _ = ctx.autotune(L.FP64, trial_steps=2)
         best, gbs = ctx.autotune(L.FP32_ACC64, trial_steps=2)
         assert gbs > 100
         assert best["cols_per_lane"] in (1, 2, 4) and best["schedule"] in (1, 2)
+        # each precision keeps its own tiling
+        assert ctx.get_tiling(L.FP64) == best64 and ctx.get_tiling(L.FP32_ACC64) == best
         am, _ = ctx.kpe_moments(24, L.FP32_ACC64)
         ctx.set_tiling(schedule=L.SCHED_BLOCKED, cols_per_lane=1)
         ref, _ = ctx.kpe_moments(24, L.FP64)
