@@ -186,6 +186,77 @@ def time_to_solution(args):
     ctx.close()
 
 
+def c1_measure(args):
+    """The 8x8x8 cell (4,608 atoms, 18,432 orbitals, sigma 0.05 A), M=512:
+    every result of the FP32 forms against FP64 -- the KPE density at the
+    neutral start and the converged SCC (each charge, E_band, mu) -- the K1
+    rate in both precisions, and dense SP2 (FP32 MFMA bulk + 2 FP64
+    iterations vs FP64 throughout) with its MFMA rates."""
+    import numpy as np
+    import largetb_amd as L
+    EV = 27.211386
+    s = L.System.quartz(8, 8, 8, sigma_ang=0.05, seed=args.seed)
+    ctx = L.Context(0)
+    ctx.load(s)
+    ctx.set_populations(None)
+    lo, hi = ctx.gershgorin()
+    ctx.rescale((hi - lo) / 2, (hi + lo) / 2)
+    out = {"workload": f"{s.nat}-atom alpha-quartz 8x8x8, {s.norb} orbitals, sigma 0.05 A, seed {args.seed}",
+           "M": 512}
+    tilings, kpe = {}, {}
+    for name, prec in (("fp64", L.FP64), ("fp32", L.FP32_ACC64)):
+        tilings[name], _ = ctx.autotune(prec, 3)
+        ctx.kpe_moments(512, prec)  # warm-up
+        ctx.set_timing(True)
+        am, _ = ctx.kpe_moments(512, prec)
+        st = ctx.kpe_stats()
+        ctx.set_timing(False)
+        d = ctx.density(512)
+        gbs = st["bytes"] / st["seconds"] / 1e9
+        kpe[name] = dict(d, am=am, gbs=gbs)
+        out[f"kpe_{name}"] = {"tiling": tilings[name], "achieved_gbs": round(gbs, 1),
+                              "frac": round(gbs / HBM_PEAK_GBS, 4), "launches": st["launches"]}
+    a, b = kpe["fp64"], kpe["fp32"]
+    out["density_fp32_vs_fp64"] = {
+        "max_atom_moment_diff": float(np.abs(a["am"] - b["am"]).max()),
+        "max_dq_e": float(2 * np.abs(a["pop"] - b["pop"]).max()),
+        "deband_ev_per_atom": abs(a["eband"] - b["eband"]) * EV / s.nat,
+        "dmu_ev": abs(a["mu_ha"] - b["mu_ha"]) * EV}
+    ctx.set_populations(s.neutral())  # SP2 at the same (neutral) start as the densities above
+    sp = {}
+    for name, kw in (("fp64", dict(max_iter=0, fp64_tail=100)), ("fp32_fp64tail", dict(max_iter=100, fp64_tail=2))):
+        ctx.sp2_dense(max_iter=2, fp64_tail=1)  # warm-up
+        t0 = time.perf_counter()
+        r = ctx.sp2_dense(**kw)
+        sp[name] = dict(r, wall=time.perf_counter() - t0)
+    for name, r in sp.items():
+        peak = 78.6 if name == "fp64" else 157.3
+        tf = r["gemm_flops"] / r["gemm_seconds"] / 1e12
+        out[f"sp2_{name}"] = {"iters": r["iters"], "fp64_iters": r["fp64_iters"], "seconds": round(r["wall"], 3),
+                              "gemm_tflops_exec": round(tf, 2), "trace_minus_nocc": r["trace"] - s.nelectrons / 2,
+                              "idem": r["idem"], "mfma_peak_tflops": peak}
+    out["sp2_fp32_vs_fp64"] = {"max_diag_diff": float(np.abs(sp["fp64"]["diag"] - sp["fp32_fp64tail"]["diag"]).max()),
+                               "max_dq_e": float(2 * np.abs(sp["fp64"]["pop"] - sp["fp32_fp64tail"]["pop"]).max())}
+    out["sp2_fp64_vs_kpe_fp64_max_dq_e"] = float(2 * np.abs(sp["fp64"]["pop"] - a["pop"]).max())
+    runs = {}
+    for name, kw in (("plain_fp64", dict(precision=L.FP64, max_iter=400)),
+                     ("mixed", dict(precision=L.FP32_ACC64, switch_tol=1e-6, max_iter=400)),
+                     ("fp32_only", dict(precision=L.FP32_ACC64, max_iter=150))):
+        t0 = time.perf_counter()
+        r = ctx.scc(512, tol=1e-6, use_graph=1, check=False, **kw)
+        runs[name] = dict(r, wall=time.perf_counter() - t0)
+    ref = runs["plain_fp64"]
+    for name, r in runs.items():
+        out[f"scc_{name}"] = {"status": r["status"], "iters": r["iters"], "fp64_iters": r["fp64_iters"],
+                              "seconds": round(r["wall"], 3),
+                              "max_dq_vs_plain_e": float(np.abs(r["q"] - ref["q"]).max()),
+                              "deband_vs_plain_ev_per_atom": abs(r["eband_fp"] - ref["eband_fp"]) * EV / s.nat,
+                              "dmu_vs_plain_ev": abs(r["mu_ha"] - ref["mu_ha"]) * EV}
+    out["command"] = "python3 -u bench.py --c1"
+    print(json.dumps(out))
+    ctx.close()
+
+
 def sp2_measure(args):
     """Dense SP2 on the 8x8x8 cell (18,432 orbitals): FP32 MFMA bulk + FP64
     tail, against the KPE density at M=512; MFMA rate over the X^2 GEMMs."""
@@ -402,6 +473,8 @@ def main():
     ap.add_argument("--no-ablation", action="store_true")
     ap.add_argument("--weights", default="", help="--tts / --segment: learned-guess weights (.npz) to use")
     ap.add_argument("--sp2", action="store_true", help="dense SP2 measurement (8x8x8 by default)")
+    ap.add_argument("--c1", action="store_true",
+                    help="8x8x8 report: FP32 forms vs FP64 (KPE density, SCC, SP2) with K1 and MFMA rates")
     ap.add_argument("--tts", action="store_true",
                     help="SCC time-to-solution: plain FP64 neutral start vs learned guess + mixed")
     ap.add_argument("--batch", type=int, default=0,
@@ -440,6 +513,9 @@ def main():
         return
     if args.sp2:
         sp2_measure(args)
+        return
+    if args.c1:
+        c1_measure(args)
         return
     if args.batch:
         batch_measure(args)

@@ -120,7 +120,9 @@ int ltb_get_hamiltonian(ltb_ctx* ctx, int* rowptr, int* col, double* val16);
 int ltb_get_gamma(ltb_ctx* ctx, int* rowptr, int* col, double* val);
 
 /* constructdeltaH (:1414-1487): v = Gamma (p - p0) on device (FP64), H_total =
- * H0 + diag(v repeated over the 4 orbitals).  p: per-spin populations [natoms]. */
+ * H0 + diag(v repeated over the 4 orbitals).  p: per-spin populations [natoms];
+ * NULL keeps the populations the context holds (neutral after loading a
+ * system, the last mixed populations after an SCC) and recomputes v. */
 int ltb_set_populations(ltb_ctx* ctx, const double* p);
 /* replace the neutral per-spin populations p0 of Delta q = p - p0 (default
  * Si 2, O 3); the reference's literal values (Si 2, O 0: :714-720) for

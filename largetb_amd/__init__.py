@@ -404,6 +404,9 @@ class Context:
         return rp, col, val
 
     def set_populations(self, p=None):
+        """Coulomb shift from per-spin populations p; None keeps the
+        populations the context holds (neutral after load, the last mixed
+        ones after an SCC)."""
         pp = _d(np.ascontiguousarray(p, np.float64)) if p is not None else None
         self._check(lib().ltb_set_populations(self._h, pp))
 
