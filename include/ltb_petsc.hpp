@@ -71,6 +71,10 @@ struct State {
   Vec mixedCharges;                     // populations H_total is built from
   std::vector<double> alphavals;
   std::string last_error;
+  // wall seconds of runKPM / computeDensityMatrix (every call returns after the
+  // device has finished): total, number of calls, and the last call
+  double t_kpm = 0, t_density = 0, t_kpm_last = 0, t_density_last = 0;
+  int n_kpm = 0, n_density = 0;
 };
 
 PetscErrorCode arrangeCoordinates(State& s);                         // :669-738

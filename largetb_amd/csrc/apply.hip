@@ -48,19 +48,10 @@ __global__ __launch_bounds__(256) void k_apply_step(const int* __restrict__ rowp
   }
 }
 
+// the recursion on device buffers: a holds t_0 (overwritten), b is scratch,
+// th receives theta; all norb x ld row-major
 template <typename T>
-int run_apply(ltb_ctx* c, int M, const double* w, int ncols, const double* X, double* Y) {
-  const int ld = (ncols + 63) / 64 * 64;
-  const size_t n = (size_t)c->norb * ld;
-  std::vector<T> hx(n, T(0));
-  for (int j = 0; j < ncols; j++)
-    for (int r = 0; r < c->norb; r++) hx[(size_t)r * ld + j] = (T)X[(size_t)j * c->norb + r];
-  T *a = nullptr, *b = nullptr;
-  double* th = nullptr;
-  LTB_HIP(c, hipMalloc(&a, n * sizeof(T)));
-  LTB_HIP(c, hipMalloc(&b, n * sizeof(T)));
-  LTB_HIP(c, hipMalloc(&th, n * sizeof(double)));
-  LTB_HIP(c, hipMemcpyAsync(a, hx.data(), n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+int apply_core(ltb_ctx* c, int M, const double* w, int ld, T* a, T* b, double* th) {
   const T* hv = sizeof(T) == 4 ? (const T*)c->d_ht32 : (const T*)c->d_ht64;
   dim3 grid(ld / 64, (c->nat + 3) / 4), block(256);
   // step 1: t1 = H t0 into b; theta = w0 t0 + w1 t1
@@ -74,6 +65,30 @@ int run_apply(ltb_ctx* c, int M, const double* w, int ncols, const double* X, do
                        cur, nxt, th, c->nat, ld, 0.0, w[m]);
   }
   LTB_HIP(c, hipGetLastError());
+  return LTB_OK;
+}
+
+template <typename T>
+int run_apply(ltb_ctx* c, int M, const double* w, int ncols, const double* X, double* Y) {
+  const int ld = (ncols + 63) / 64 * 64;
+  const size_t n = (size_t)c->norb * ld;
+  std::vector<T> hx(n, T(0));
+  for (int j = 0; j < ncols; j++)
+    for (int r = 0; r < c->norb; r++) hx[(size_t)r * ld + j] = (T)X[(size_t)j * c->norb + r];
+  T *a = nullptr, *b = nullptr;
+  double* th = nullptr;
+  LTB_HIP(c, hipMalloc(&a, n * sizeof(T)));
+  LTB_HIP(c, hipMalloc(&b, n * sizeof(T)));
+  LTB_HIP(c, hipMalloc(&th, n * sizeof(double)));
+  LTB_HIP(c, hipMemcpyAsync(a, hx.data(), n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  int st = apply_core<T>(c, M, w, ld, a, b, th);
+  if (st) {
+    (void)hipStreamSynchronize(c->stream);
+    hipFree(a);
+    hipFree(b);
+    hipFree(th);
+    return st;
+  }
   std::vector<double> ht(n);
   LTB_HIP(c, hipMemcpyAsync(ht.data(), th, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   LTB_HIP(c, hipStreamSynchronize(c->stream));
@@ -86,6 +101,11 @@ int run_apply(ltb_ctx* c, int M, const double* w, int ncols, const double* X, do
 }
 
 }  // namespace
+
+int kpe_apply_device(ltb_ctx* c, int M, const double* w, int ld, void* a, void* b, double* theta, int precision) {
+  if (precision == LTB_FP32_ACC64) return apply_core<float>(c, M, w, ld, (float*)a, (float*)b, theta);
+  return apply_core<double>(c, M, w, ld, (double*)a, (double*)b, theta);
+}
 
 extern "C" int ltb_kpe_apply(ltb_ctx* c, int M, const double* w, int ncols, const double* X,
                              double* Y, int precision) {

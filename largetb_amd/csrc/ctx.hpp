@@ -102,6 +102,9 @@ int kpe_moments_impl(ltb_ctx* c, int M, int precision);
 int kpe_block_moments_impl(ltb_ctx* c, int M, int nvec, int precision, const double* d_r2, double* d_momcol,
                            int ld);
 int kpe_block_width(ltb_ctx* c, int precision, int nvec);
+// getKPE on a device block: a holds t_0 (norb x ld, row-major, overwritten), b
+// is scratch of the same size, theta (FP64, norb x ld) receives the result
+int kpe_apply_device(ltb_ctx* c, int M, const double* w, int ld, void* a, void* b, double* theta, int precision);
 int kpe_time_steps(ltb_ctx* c, int precision, int steps, double* seconds, double* bytes);
 int kpe_resolved_tiling(ltb_ctx* c, int precision, ltb_kpe_tiling* out);
 typedef ltb_kpe_tiling KpeTiling;

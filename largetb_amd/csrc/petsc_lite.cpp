@@ -4,6 +4,7 @@
 #include "../../include/ltb_petsc.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 
@@ -45,6 +46,27 @@ int diag_block(State& s, int depth, int j0, int j1, std::vector<double>& out) {
 }
 
 std::string dump_path(const State& s, const char* file) { return s.opt.dump_dir + "/" + file; }
+
+// wall time of one phase call, added to the state's totals on every return path
+struct PhaseTimer {
+  double& total;
+  double& last;
+  int& calls;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  PhaseTimer(double& t, double& l, int& n) : total(t), last(l), calls(n) {}
+  ~PhaseTimer() {
+    last = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    total += last;
+    calls++;
+  }
+};
+
+// the reference prints these two after the first density (main :1800-1812)
+void print_phase_times(const State& s) {
+  if (!s.opt.verbose) return;
+  std::printf("time KPM : %g\n", s.t_kpm_last);
+  std::printf("time SP2 : %g\n", s.t_density_last);
+}
 
 // getKPE's coefficients c_m(mu) g_m (:1094-1132): from the bisected mu of the
 // exact mode, or at runKPM's DOS mu in stochastic mode
@@ -159,6 +181,7 @@ PetscErrorCode GershGorinParallel(State& s, double* mn, double* mx) {
 }
 
 PetscErrorCode runKPM(State& s) {
+  PhaseTimer timer(s.t_kpm, s.t_kpm_last, s.n_kpm);
   // bounds and rescale as :1541-1560
   double mn, mx;
   CHK(s, GershGorinParallel(s, &mn, &mx));
@@ -270,6 +293,7 @@ PetscErrorCode computeDiagonalElements(State& s) {
 }
 
 PetscErrorCode computeDensityMatrix(State& s) {
+  PhaseTimer timer(s.t_density, s.t_density_last, s.n_density);
   s.alphavals.assign(std::max(1, s.opt.maxsp2), 0.0);
   CHK(s, computeAlphas(s));
   CHK(s, computeDiagonalElements(s));
@@ -328,6 +352,7 @@ PetscErrorCode scf_literal(State& s, int* iterations, double* delta) {
   CHK(s, constructdeltaH(s));
   CHK(s, runKPM(s));
   CHK(s, computeDensityMatrix(s));
+  print_phase_times(s);
   s.previousCharges = s.chargeOnAtom;
   int it = 0;
   double d = INFINITY;
@@ -358,6 +383,7 @@ PetscErrorCode scf(State& s, int* iterations, double* delta) {
     CHK(s, constructdeltaH(s));
     CHK(s, runKPM(s));
     CHK(s, computeDensityMatrix(s));
+    if (it == 0) print_phase_times(s);
     d = 0;
     for (int i = 0; i < s.Natoms; i++) d = std::max(d, 2.0 * std::fabs(s.chargeOnAtom[i] - s.mixedCharges[i]));
     if (s.opt.verbose) std::printf("scfiter : %d , the value of the scfDelta is %e \n", it, d);  // :1835

@@ -81,9 +81,11 @@ int main(int argc, char** argv) {
   }
   std::printf("{\"status\": %d, \"atoms\": %d, \"iterations\": %d, \"delta\": %.3e, \"seconds\": %.3f, "
               "\"mu_ha\": %.12f, \"eband_ha\": %.12f, \"total_charge\": %.12f, "
-              "\"q_si\": [%.9f, %.9f], \"q_o\": [%.9f, %.9f]}\n",
+              "\"q_si\": [%.9f, %.9f], \"q_o\": [%.9f, %.9f], "
+              "\"runKPM\": {\"calls\": %d, \"seconds\": %.4f}, "
+              "\"computeDensityMatrix\": {\"calls\": %d, \"seconds\": %.4f}}\n",
               st, s.Natoms, iters, delta, t_scf, s.mu_ha, s.eband, tot, qsi_min, qsi_max,
-              qo_min, qo_max);
+              qo_min, qo_max, s.n_kpm, s.t_kpm, s.n_density, s.t_density);
   // full charge vector for checkers
   if (const char* path = std::getenv("LTB_SCF_CHARGES")) {
     if (FILE* f = std::fopen(path, "w")) {

@@ -9,9 +9,14 @@
 //     diagP = sum_r r .* (P r) ./ sum_r r .* r.
 // The vectors come from the SplitMix64 -> Box-Muller stream of a seed (the
 // reference seeds std::mt19937 from std::random_device, so its draws are not
-// reproducible; the estimator is the same).  The moments of all vectors are
+// reproducible; the estimator is the same).  The stream is random access
+// (normal i uses draws 2(i/2)+1 and 2(i/2)+2), so the seeded entry points draw
+// their vectors on the device, straight into the recursion's row-major block;
+// they equal ltb_random_vectors' host draws up to the last-ulp differences of
+// the device and host log / sqrt / sin / cos.  The moments of all vectors are
 // one block recursion on the device (the exact-trace kernels with explicit
-// start vectors and moment doubling); P r is getKPE on the vector block.
+// start vectors and moment doubling); P r is getKPE on the vector block, and
+// the diagonal estimate is formed on the device as well.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -38,6 +43,68 @@ void to_rowmajor(const double* R, int nvec, int norb, int ld, std::vector<T>& ou
   out.assign((size_t)norb * ld, T(0));
   for (int cidx = 0; cidx < nvec; cidx++)
     for (int j = 0; j < norb; j++) out[(size_t)j * ld + cidx] = (T)R[(size_t)cidx * norb + j];
+}
+
+// normal number i of the stream of `seed` (ltb::gaussian_fill's sequence):
+// uniforms ((z >> 11) + 1) 2^-53 of SplitMix64 draws 2p+1 and 2p+2, p = i/2,
+// then r cos t for even i and r sin t for odd i
+__device__ __forceinline__ double gauss_at(unsigned long long seed, unsigned long long i) {
+  const unsigned long long p = i >> 1;
+  double u[2];
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    unsigned long long z = seed + (2 * p + 1 + q) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    u[q] = (double)((z >> 11) + 1) * 0x1.0p-53;
+  }
+  const double r = sqrt(-2.0 * log(u[0])), t = 2.0 * kPi * u[1];
+  return (i & 1) ? r * sin(t) : r * cos(t);
+}
+
+// x[j][k] = vector k's orbital j = normal k*norb + j of the stream, in the
+// recursion's precision; columns nvec..ld-1 zero
+template <typename T>
+__global__ __launch_bounds__(256) void k_gauss_block(unsigned long long seed, int norb, int nvec, int ld,
+                                                     T* __restrict__ x) {
+  const int k = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (j >= norb || k >= ld) return;
+  x[(size_t)j * ld + k] = k < nvec ? (T)gauss_at(seed, (unsigned long long)k * norb + j) : T(0);
+}
+
+// |r_k|^2 of the vectors as the recursion holds them (rounded to T)
+template <typename T>
+__global__ __launch_bounds__(256) void k_gauss_norm2(unsigned long long seed, int norb, double* __restrict__ r2) {
+  const int k = blockIdx.x;
+  double s = 0;
+  for (int j = threadIdx.x; j < norb; j += 256) {
+    const double x = (double)(T)gauss_at(seed, (unsigned long long)k * norb + j);
+    s += x * x;
+  }
+  __shared__ double sh[256];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) r2[k] = sh[0];
+}
+
+// diagP_j = sum_k r_jk (P r)_jk / sum_k r_jk^2 (computeDiagonalElements :1338-1392)
+__global__ __launch_bounds__(256) void k_stoch_diag(unsigned long long seed, int norb, int nvec, int ld,
+                                                    const double* __restrict__ theta, double* __restrict__ diagP) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= norb) return;
+  double num = 0, den = 0;
+  for (int k = 0; k < nvec; k++) {
+    const double r = gauss_at(seed, (unsigned long long)k * norb + j);
+    num += r * theta[(size_t)j * ld + k];
+    den += r * r;
+  }
+  diagP[j] = num / den;
 }
 
 }  // namespace
@@ -101,31 +168,67 @@ int ltb_random_vectors(unsigned long long seed, int nvec, int n, double* R) {
 }
 
 int ltb_kpe_stochastic(ltb_ctx* c, int M, int nvec, unsigned long long seed, int precision, double* trace) {
-  if (!c || nvec < 1) return LTB_ERR_ARG;
+  if (!c || M < 1 || nvec < 1 || (precision != LTB_FP64 && precision != LTB_FP32_ACC64)) return LTB_ERR_ARG;
   if (c->nat <= 0) return set_err(c, LTB_ERR_STATE, "no system loaded");
-  std::vector<double> R((size_t)nvec * c->norb);
-  ltb::gaussian_fill(seed, R.size(), R.data());
-  return ltb_kpe_block_moments(c, M, nvec, R.data(), precision, nullptr, trace);
+  const int norb = c->norb, ld = kpe_block_width(c, precision, nvec);
+  const bool f64 = precision == LTB_FP64;
+  int st;
+  if ((st = ensure_buffer(c, &c->d_xa, &c->xabytes, (size_t)norb * ld * (f64 ? 8 : 4)))) return st;
+  if ((st = ensure_buffer(c, &c->d_xb, &c->xbbytes, (size_t)norb * ld * (f64 ? 8 : 4)))) return st;
+  DevBuf d_r2, d_mom;
+  LTB_HIP(c, hipMalloc(&d_r2.p, sizeof(double) * ld));
+  LTB_HIP(c, hipMalloc(&d_mom.p, sizeof(double) * (size_t)(M + 1) * ld));
+  const dim3 grid(ld / 64, (norb + 3) / 4);
+  if (f64) {
+    hipLaunchKernelGGL(k_gauss_block<double>, grid, dim3(256), 0, c->stream, seed, norb, nvec, ld, (double*)c->d_xa);
+    hipLaunchKernelGGL(k_gauss_norm2<double>, dim3(nvec), dim3(256), 0, c->stream, seed, norb, (double*)d_r2.p);
+  } else {
+    hipLaunchKernelGGL(k_gauss_block<float>, grid, dim3(256), 0, c->stream, seed, norb, nvec, ld, (float*)c->d_xa);
+    hipLaunchKernelGGL(k_gauss_norm2<float>, dim3(nvec), dim3(256), 0, c->stream, seed, norb, (double*)d_r2.p);
+  }
+  LTB_HIP(c, hipGetLastError());
+  if ((st = kpe_block_moments_impl(c, M, nvec, precision, (const double*)d_r2.p, (double*)d_mom.p, ld))) return st;
+  std::vector<double> mom((size_t)(M + 1) * ld);
+  LTB_HIP(c, hipMemcpyAsync(mom.data(), d_mom.p, sizeof(double) * mom.size(), hipMemcpyDeviceToHost, c->stream));
+  LTB_HIP(c, hipStreamSynchronize(c->stream));
+  if (trace)
+    for (int m = 0; m <= M; m++) {
+      double s = 0;
+      for (int k = 0; k < nvec; k++) s += mom[(size_t)m * ld + k];
+      trace[m] = s;
+    }
+  return LTB_OK;
 }
 
 int ltb_density_stochastic(ltb_ctx* c, int M, const double* w, int nvec, unsigned long long seed,
                            int precision, double* diagP) {
-  if (!c || !w || !diagP || nvec < 1) return LTB_ERR_ARG;
+  if (!c || M < 1 || !w || !diagP || nvec < 1 || (precision != LTB_FP64 && precision != LTB_FP32_ACC64))
+    return LTB_ERR_ARG;
   if (c->nat <= 0) return set_err(c, LTB_ERR_STATE, "no system loaded");
-  const int norb = c->norb;
-  std::vector<double> R((size_t)nvec * norb), Y((size_t)nvec * norb);
-  ltb::gaussian_fill(seed, R.size(), R.data());
-  int st = ltb_kpe_apply(c, M, w, nvec, R.data(), Y.data(), precision);
-  if (st) return st;
-  for (int j = 0; j < norb; j++) {
-    double num = 0, den = 0;
-    for (int k = 0; k < nvec; k++) {
-      const double r = R[(size_t)k * norb + j];
-      num += r * Y[(size_t)k * norb + j];
-      den += r * r;
-    }
-    diagP[j] = num / den;
+  const int norb = c->norb, ld = (nvec + 63) / 64 * 64;
+  const bool f64 = precision == LTB_FP64;
+  const size_t n = (size_t)norb * ld;
+  DevBuf a, b, th, d;
+  LTB_HIP(c, hipMalloc(&a.p, n * (f64 ? 8 : 4)));
+  LTB_HIP(c, hipMalloc(&b.p, n * (f64 ? 8 : 4)));
+  LTB_HIP(c, hipMalloc(&th.p, n * sizeof(double)));
+  LTB_HIP(c, hipMalloc(&d.p, sizeof(double) * norb));
+  const dim3 grid(ld / 64, (norb + 3) / 4);
+  if (f64)
+    hipLaunchKernelGGL(k_gauss_block<double>, grid, dim3(256), 0, c->stream, seed, norb, nvec, ld, (double*)a.p);
+  else
+    hipLaunchKernelGGL(k_gauss_block<float>, grid, dim3(256), 0, c->stream, seed, norb, nvec, ld, (float*)a.p);
+  LTB_HIP(c, hipGetLastError());
+  int st = kpe_apply_device(c, M, w, ld, a.p, b.p, (double*)th.p, precision);
+  if (st) {
+    (void)hipStreamSynchronize(c->stream);  // the buffers are freed on return
+    return st;
   }
+  hipLaunchKernelGGL(k_stoch_diag, dim3((norb + 255) / 256), dim3(256), 0, c->stream, seed, norb, nvec, ld,
+                     (const double*)th.p, (double*)d.p);
+  LTB_HIP(c, hipGetLastError());
+  LTB_HIP(c, hipMemcpyAsync(diagP, d.p, sizeof(double) * norb, hipMemcpyDeviceToHost, c->stream));
+  LTB_HIP(c, hipStreamSynchronize(c->stream));
   return LTB_OK;
 }
 

@@ -33,6 +33,10 @@ def test_scf_driver_matches_oracle(tmp_path):
                      "--max-scf", "300"], tmp_path)
     assert r.returncode == 0, r.stderr
     assert "scfiter : 0 , the value of the scfDelta is" in r.stdout
+    # the reference's phase timers, and one runKPM + density per iteration
+    assert "time KPM : " in r.stdout and "time SP2 : " in r.stdout
+    assert out["runKPM"]["calls"] == out["iterations"] == out["computeDensityMatrix"]["calls"]
+    assert out["runKPM"]["seconds"] > 0
     o = O.System.quartz(2, 2, 2, sigma_ang=0.05, seed=1)
     ref = o.scc(200, tol=1e-6, maxit=300)
     assert abs(out["iterations"] - ref["iters"]) <= 1

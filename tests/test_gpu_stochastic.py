@@ -55,7 +55,10 @@ def test_gaussian_block_matches_dense(ctx, c0, prec, tol):
     ref = cheb_dense(Ht, R, M)
     assert np.abs(mom - ref).max() < tol * s.norb
     assert np.allclose(tr, ref.sum(0), rtol=0, atol=tol * s.norb * nvec)
-    assert np.array_equal(ctx.stochastic_moments(M, nvec, seed=7, precision=prec), tr)
+    # the seeded entry point draws the same stream on the device (device and host
+    # libm may differ in the last ulp), so its trace equals the host vectors' one
+    st = ctx.stochastic_moments(M, nvec, seed=7, precision=prec)
+    assert np.allclose(st, tr, rtol=0, atol=(1e-12 if prec == L.FP64 else tol) * s.norb * nvec)
 
 
 def test_stochastic_trace_estimate(ctx, c0):
