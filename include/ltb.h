@@ -265,7 +265,9 @@ int ltb_kpe_block_moments(ltb_ctx* ctx, int M, int nvec, const double* R, int pr
 /* nvec x n standard normal numbers (vector-major) from the SplitMix64 ->
  * Box-Muller stream of seed -- the vectors the two calls below use */
 int ltb_random_vectors(unsigned long long seed, int nvec, int n, double* R);
-/* runKPM's stochastic moments (:1607-1633) with nvec seeded Gaussian vectors */
+/* runKPM's stochastic moments (:1607-1633) with nvec seeded Gaussian vectors,
+ * drawn on the device (the ltb_random_vectors stream up to last-ulp libm
+ * differences) */
 int ltb_kpe_stochastic(ltb_ctx* ctx, int M, int nvec, unsigned long long seed, int precision, double* trace);
 /* computeDiagonalElements, random mode (:1338-1392): diagP = sum_r r.*(P r) ./
  * sum_r r.*r, P r = sum_m w[m] T_m(H~) r (getKPE on the block, w as for
