@@ -481,8 +481,8 @@ def main():
                     help="batched SCC over this many uneven cells (M=256 unless --M is given)")
     ap.add_argument("--no-graph", action="store_true",
                     help="batch mode: eager iterations only (for kernel tracing)")
-    ap.add_argument("--batch-seq", type=int, default=32,
-                    help="cells re-solved one at a time for the comparison (0: all)")
+    ap.add_argument("--batch-seq", type=int, default=0,
+                    help="cells re-solved one at a time for the comparison (0: all; fewer: a sample scaled to all)")
     ap.add_argument("--segment", choices=["plain", "mixed"],
                     help="one resumable segment of an SCC solve (plain FP64 or FP32 bulk + FP64 finish)")
     ap.add_argument("--start", choices=["neutral", "learned"], default="neutral")
