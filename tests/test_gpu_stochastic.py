@@ -88,3 +88,26 @@ def test_stochastic_diagonal(ctx, c0):
     assert np.abs(got - ref).max() < 1e-9
     # the estimator is unbiased for diag(P) only as nvec grows; the sum is not exact
     assert abs(got.sum() - d["nelec"] / 2) < 0.1 * s.norb
+
+
+@pytest.mark.parametrize("prec,tol", [(L.FP64, 1e-11), (L.FP32_ACC64, 1e-5)])
+def test_device_draws_match_host_stream_at_size(ctx, prec, tol):
+    """18,432 orbitals: the seeded entry points (vectors drawn on the device,
+    indices up to nvec * norb) against the same stream drawn on the host
+    (ltb_random_vectors) through the host-vector entry points."""
+    s = L.System.quartz(8, 8, 8, sigma_ang=0.05, seed=1)
+    ctx.load(s)
+    ctx.set_populations(None)
+    lo, hi = ctx.gershgorin()
+    ctx.rescale((hi - lo) / 2, (hi + lo) / 2)
+    M, nvec, seed = 48, 40, 2024
+    R = L.random_vectors(seed, nvec, s.norb)
+    _, tr = ctx.block_moments(M, R, prec)
+    st = ctx.stochastic_moments(M, nvec, seed=seed, precision=prec)
+    assert np.abs(st - tr).max() < tol * s.norb * nvec
+    # the random-mode diagonal: device draws + device estimator vs host vectors + getKPE
+    w = ctx.kpe_weights_at(M, 0.5 * (lo + hi))
+    got = ctx.density_stochastic(M, w, nvec, seed=seed, precision=prec)
+    Y = ctx.kpe_apply(M, w, R, prec)
+    ref = np.sum(R * Y, 0) / np.sum(R * R, 0)
+    assert np.abs(got - ref).max() < 10 * tol
