@@ -191,9 +191,10 @@ PetscErrorCode runKPM(State& s) {
   if (s.opt.nrandoms > 0) {
     // stochastic moments of the trace, DOS and mu from its area (:1562-1701).
     // The reference then narrows [a, b] to where the DOS exceeds 0.001 per
-    // orbital (:1702-1712); that can cut the band edges, and T_m of an
-    // eigenvalue outside [-1, 1] grows like cosh(m acosh|x|), so the
-    // Gershgorin frame is kept and the narrowed bounds are only reported.
+    // orbital and rescales H with it (:1702-1712).  That can cut the band
+    // edges, and T_m of an eigenvalue outside [-1, 1] grows like
+    // cosh(m acosh|x|), so the Gershgorin frame is kept -- except in
+    // reference-literal mode, which narrows as the reference does.
     const int nm = s.opt.dos_moments;
     std::vector<double> tr(nm);
     CHK(s, ltb_kpe_stochastic(s.ctx, nm - 1, s.opt.nrandoms, s.opt.seed + 1000003ULL * ++s.draws, s.opt.precision,
@@ -206,6 +207,11 @@ PetscErrorCode runKPM(State& s) {
     if (!s.opt.dump_dir.empty())
       CHKIO(s, ltb_write_dos(dump_path(s, "dosfile.txt").c_str(), 2 * nm, s.dosEnergies.data(),
                              s.dosValues.data()));
+    if (s.opt.reference_literal && hi > lo) {
+      s.a = (hi - lo) / 2;
+      s.b = (hi + lo) / 2;
+      CHK(s, ltb_rescale(s.ctx, s.a, s.b));
+    }
     s.mu_ha = mu;
     s.muval = (mu - s.b) / s.a;
     s.eband = 0;
