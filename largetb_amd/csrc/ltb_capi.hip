@@ -945,6 +945,7 @@ int ltb_get_kpe_tiling(ltb_ctx* c, int precision, ltb_kpe_tiling* out) {
   int st = need_system(c);
   if (st) return st;
   if (!out) return set_err(c, LTB_ERR_ARG, "null output");
+  if (precision != LTB_FP64 && precision != LTB_FP32_ACC64) return set_err(c, LTB_ERR_ARG, "bad precision");
   return kpe_resolved_tiling(c, precision, out);
 }
 
@@ -952,6 +953,7 @@ int ltb_kpe_autotune(ltb_ctx* c, int precision, int trial_steps, ltb_kpe_tiling*
                      double* best_gbs) {
   int st = need_system(c);
   if (st) return st;
+  if (precision != LTB_FP64 && precision != LTB_FP32_ACC64) return set_err(c, LTB_ERR_ARG, "bad precision");
   if (trial_steps < 1) trial_steps = 3;
   ltb_kpe_tiling& cur = c->tiling[precision == LTB_FP64 ? 1 : 0];  // only this precision's tiling is tuned
   const ltb_kpe_tiling saved = cur;
