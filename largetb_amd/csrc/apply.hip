@@ -68,6 +68,16 @@ int apply_core(ltb_ctx* c, int M, const double* w, int ld, T* a, T* b, double* t
   return LTB_OK;
 }
 
+// device buffers of one call, freed on every return path once the stream has
+// drained (hipFree waits for the device)
+struct CallBuffers {
+  void* p[3] = {nullptr, nullptr, nullptr};
+  ~CallBuffers() {
+    for (void* q : p)
+      if (q) (void)hipFree(q);
+  }
+};
+
 template <typename T>
 int run_apply(ltb_ctx* c, int M, const double* w, int ncols, const double* X, double* Y) {
   const int ld = (ncols + 63) / 64 * 64;
@@ -75,26 +85,22 @@ int run_apply(ltb_ctx* c, int M, const double* w, int ncols, const double* X, do
   std::vector<T> hx(n, T(0));
   for (int j = 0; j < ncols; j++)
     for (int r = 0; r < c->norb; r++) hx[(size_t)r * ld + j] = (T)X[(size_t)j * c->norb + r];
-  T *a = nullptr, *b = nullptr;
-  double* th = nullptr;
-  LTB_HIP(c, hipMalloc(&a, n * sizeof(T)));
-  LTB_HIP(c, hipMalloc(&b, n * sizeof(T)));
-  LTB_HIP(c, hipMalloc(&th, n * sizeof(double)));
+  CallBuffers buf;
+  LTB_HIP(c, hipMalloc(&buf.p[0], n * sizeof(T)));
+  LTB_HIP(c, hipMalloc(&buf.p[1], n * sizeof(T)));
+  LTB_HIP(c, hipMalloc(&buf.p[2], n * sizeof(double)));
+  T* a = (T*)buf.p[0];
+  T* b = (T*)buf.p[1];
+  double* th = (double*)buf.p[2];
   LTB_HIP(c, hipMemcpyAsync(a, hx.data(), n * sizeof(T), hipMemcpyHostToDevice, c->stream));
   int st = apply_core<T>(c, M, w, ld, a, b, th);
   if (st) {
     (void)hipStreamSynchronize(c->stream);
-    hipFree(a);
-    hipFree(b);
-    hipFree(th);
     return st;
   }
   std::vector<double> ht(n);
   LTB_HIP(c, hipMemcpyAsync(ht.data(), th, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   LTB_HIP(c, hipStreamSynchronize(c->stream));
-  hipFree(a);
-  hipFree(b);
-  hipFree(th);
   for (int j = 0; j < ncols; j++)
     for (int r = 0; r < c->norb; r++) Y[(size_t)j * c->norb + r] = ht[(size_t)r * ld + j];
   return LTB_OK;
