@@ -38,3 +38,17 @@ def test_host_entry_points_reject_bad_arguments(tmp_path):
         L.random_vectors(1, 0, 5)
     with pytest.raises(L.LtbError):
         L.write_dos(tmp_path / "no" / "dir" / "dos.txt", [0.0], [1.0])
+
+
+def test_bench_traffic_comes_from_the_committed_counter_summary():
+    """roofline.traffic: the scaled FETCH_SIZE + WRITE_SIZE figure of the profiled
+    tiling, and None for any other workload or tiling."""
+    sys.path.insert(0, ROOT)
+    import bench
+    t = dict(chunk_cols=4096, cols_per_lane=4, schedule=2, blocks_per_cu=2, nontemporal=1, order=2)
+    v = bench.measured_traffic([12, 12, 12], 512, t, "fp32")
+    assert v is not None and 3.0e9 < v < 5.0e9        # algorithmic 3.062 GB per launch
+    assert bench.measured_traffic([8, 8, 8], 512, t, "fp32") is None
+    assert bench.measured_traffic([12, 12, 12], 512, dict(t, cols_per_lane=2), "fp32") is None
+    assert bench.measured_traffic([12, 12, 12], 512, t, "fp64") is None
+
