@@ -258,6 +258,20 @@ def test_nonfinite_populations_are_reported(ctx, c0):
     ctx.set_populations(o.p0)
 
 
+def test_largest_cell_properties(ctx):
+    """27,648 atoms (110,592 orbitals, the largest single-GPU cell): many chunks in
+    both precisions, sum rules, and FP32 against FP64 per atom moment."""
+    s = L.System.quartz(16, 16, 12, sigma_ang=0.08, seed=1)
+    prepare(ctx, s)
+    M = 16
+    am64, tot64 = ctx.kpe_moments(M, L.FP64)
+    am32, tot32 = ctx.kpe_moments(M, L.FP32_ACC64)
+    assert tot64[0] == s.norb and np.all(am64[0] == 4.0)
+    assert np.all(np.abs(am64) <= 4 + 1e-9)           # |[T_m]_jj| <= 1 for a spectrum inside [-1, 1]
+    assert np.abs(am32 - am64).max() < 5e-6
+    assert np.abs(tot64 - am64.sum(1)).max() < 1e-8 * s.norb
+
+
 def test_c1_size_properties(ctx):
     """4,608 atoms (18,432 orbitals): FP32 vs FP64 and sum rules at full size."""
     s = L.System.quartz(8, 8, 8, sigma_ang=0.05, seed=1)
