@@ -153,3 +153,24 @@ def test_batch_tilings_agree_and_autotune(batch):
     assert gbs > 0 and best["atoms_per_wave"] in (4, 8, 16) and best["blocks_per_cu"] in (2, 4, 8)
     r = batch.scc(200, precision=L.FP64, tol=1e-6, max_iter=300)
     assert np.abs(r["q"] - base["q"]).max() < 1e-10
+
+
+def test_batch_covers_every_column_group_shape(ctx):
+    """Cells whose orbital counts leave 0, 64, 128 and 256 columns for the tail
+    group (4x4x4: 2304 = 9 x 256; 2x2x2: 288; 2x3x4: 864; 3x3x2: 648), in both
+    precisions, against their own single-structure solves."""
+    cells = [(4, 4, 4, 0.05, 21), (2, 2, 2, 0.08, 22), (2, 3, 4, 0.05, 23), (3, 3, 2, 0.12, 24)]
+    systems = [L.System.quartz(l, m, n, sigma_ang=sg, seed=sd) for l, m, n, sg, sd in cells]
+    assert sorted((s.norb % 256) for s in systems) == [0, 32, 96, 136]
+    singles = []
+    for s in systems:
+        ctx.load(s)
+        singles.append(ctx.scc(128, precision=L.FP64, tol=1e-6, max_iter=300))
+    with L.Batch(ctx, systems) as b:
+        r64 = b.scc(128, precision=L.FP64, tol=1e-6, max_iter=300)
+        rmx = b.scc(128, precision=L.FP32_ACC64, switch_tol=1e-6, tol=1e-6, max_iter=300)
+        q64, qmx = b.split(r64["q"]), b.split(rmx["q"])
+    assert r64["status"] == 0 and rmx["status"] == 0
+    for k, g in enumerate(singles):
+        assert np.abs(q64[k] - g["q"]).max() < 1e-8, k
+        assert np.abs(qmx[k] - g["q"]).max() < 1e-6, k
