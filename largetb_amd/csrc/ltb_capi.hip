@@ -343,10 +343,10 @@ int grid_for(int n, int bs) { return (n + bs - 1) / bs; }
 // Fermi quadrature / kernel setup for (M, K)
 int ensure_fermi(ltb_ctx* c, int M, int K) {
   if (c->fd_M == M && c->fd_K == K && c->d_W) return LTB_OK;
-  hipFree(c->d_W);
-  hipFree(c->d_X);
-  hipFree(c->d_g);
-  hipFree(c->d_w);
+  (void)hipFree(c->d_W);
+  (void)hipFree(c->d_X);
+  (void)hipFree(c->d_g);
+  (void)hipFree(c->d_w);
   c->d_W = c->d_X = c->d_g = c->d_w = nullptr;
   LTB_HIP(c, hipMalloc(&c->d_W, sizeof(double) * std::max(K, 1)));
   LTB_HIP(c, hipMalloc(&c->d_X, sizeof(double) * std::max(K, 1)));
@@ -426,7 +426,7 @@ void free_all(ltb_ctx* c) {
                   c->d_v, c->d_sc, c->d_red, c->d_xa, c->d_xb, c->d_part, c->d_mu1, c->d_mom,
                   c->d_W, c->d_X, c->d_g, c->d_c, c->d_w, c->d_sa, c->d_sb, c->d_momorb, c->d_visit};
   for (void* p : ptrs)
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
 }
 
 }  // namespace
@@ -438,7 +438,7 @@ int set_err(ltb_ctx* c, int code, const std::string& msg) {
 
 int ensure_buffer(ltb_ctx* c, void** p, size_t* cap, size_t bytes) {
   if (*p && *cap >= bytes) return LTB_OK;
-  if (*p) hipFree(*p);
+  if (*p) (void)hipFree(*p);
   *p = nullptr;
   *cap = 0;
   hipError_t e = hipMalloc(p, bytes);
@@ -555,23 +555,29 @@ int ltb_create(int device, ltb_ctx** out) {
   if (hipMalloc(&c->d_sc, sizeof(DevScalars)) != hipSuccess ||
       hipMalloc(&c->d_red, sizeof(double) * 3 * kRedBlocks) != hipSuccess) {
     free_all(c);
-    hipStreamDestroy(c->stream);
+    (void)hipStreamDestroy(c->stream);
     delete c;
     g_sys_err = "allocation failed";
     return LTB_ERR_NOMEM;
   }
-  hipMemset(c->d_sc, 0, sizeof(DevScalars));
+  if (hipMemset(c->d_sc, 0, sizeof(DevScalars)) != hipSuccess) {
+    free_all(c);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    g_sys_err = "device scalar initialisation failed";
+    return LTB_ERR_DEVICE;
+  }
   *out = c;
   return LTB_OK;
 }
 
 void ltb_destroy(ltb_ctx* c) {
   if (!c) return;
-  hipSetDevice(c->device);
-  hipStreamSynchronize(c->stream);
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
   free_all(c);
-  for (hipEvent_t e : c->events) hipEventDestroy(e);
-  hipStreamDestroy(c->stream);
+  for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
