@@ -6,9 +6,15 @@ its definition is the project's:
   descriptor: 2 x 32 radial Gaussians per neighbour species and 16 angular
   terms, cutoff 6 A; network per centre species 80-64-64-64-1 with SiLU;
   q_raw = shift_c + y; mean projection q = q_raw - (sum q_raw - Q)/n.
-Pins: backward against central finite differences of forward (test), the
-angular term against a direct triple loop on a small cell, the projection
-against its definition (sum q = Q).
+Pins (tests/test_oracle_mlp.py): backward against central finite
+differences of forward; the angular term against a direct triple loop on a
+small cell; the radial block by the Poisson sum rules of unit-spacing
+Gaussians (sum, first and second moment over the centres) on a cluster and
+by dimer closed forms (fc = 1/2 at rc/2, 3/4 at rc/3, species blocks, zero
+at the cutoff); the image enumeration by a 1x1x1 cell (images two cells
+away) equalling its 3x3x3 tiling and a +-4 image count; translation,
+rotation and permutation invariance; the projection against its definition
+(sum q = Q).
 """
 import numpy as np
 
