@@ -175,6 +175,11 @@ int ltb_set_kpe_orbital_moments(ltb_ctx* ctx, int on);
  * as getKPE applies them (:1134-1166) -- input of ltb_kpe_apply */
 int ltb_kpe_weights(ltb_ctx* ctx, int M, double* w);
 int ltb_density_diag(ltb_ctx* ctx, double* diagP);
+/* the per-orbital moments themselves: out[m*(j1-j0) + j - j0] = [T_m(H~)]_jj,
+ * m = 0..M, j0 <= j < j1, of the last ltb_kpe_moments call (with M, and with
+ * per-orbital moments on) -- the diagonal element getKPE's exact mode reads
+ * per unit vector (:1361-1371) before the weights are applied */
+int ltb_get_orbital_moments(ltb_ctx* ctx, int M, int j0, int j1, double* out);
 /* the weights w[m] (as ltb_kpe_weights) at a given chemical potential mu_ha
  * (Hartree; e.g. runKPM's mu from the DOS, :1681-1712) in the frame of the
  * last ltb_rescale: Fermi-Dirac at kT (Hartree) by K-node quadrature, or the

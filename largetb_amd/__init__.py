@@ -129,6 +129,7 @@ SIGNATURES = {
     "ltb_kpe_stats": (C.c_int, [VP, PI, PD, PD]),
     "ltb_set_kpe_orbital_moments": (C.c_int, [VP, C.c_int]),
     "ltb_density_diag": (C.c_int, [VP, PD]),
+    "ltb_get_orbital_moments": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, PD]),
     "ltb_kpe_weights": (C.c_int, [VP, C.c_int, PD]),
     "ltb_kpe_weights_at": (C.c_int, [VP, C.c_int, C.c_double, C.c_double, C.c_int, PD]),
     "ltb_set_kpe_timing": (C.c_int, [VP, C.c_int]),
@@ -495,6 +496,12 @@ class Context:
         w = np.zeros(M)
         self._check(lib().ltb_kpe_weights(self._h, M, _d(w)))
         return w
+
+    def orbital_moments(self, M, j0, j1):
+        """[T_m(H~)]_jj for m = 0..M and j0 <= j < j1: [M+1, j1-j0]"""
+        out = np.zeros((M + 1) * (j1 - j0))
+        self._check(lib().ltb_get_orbital_moments(self._h, M, j0, j1, _d(out)))
+        return out.reshape(M + 1, j1 - j0)
 
     def density_diag(self):
         d = np.zeros(self.norb)

@@ -74,27 +74,23 @@ __global__ __launch_bounds__(256) void k_batch_mark(BatchTask* __restrict__ task
 // sum the 4 waves' per-column partials in LDS (FP64) and write one row pair;
 // two LDS buffers used in turn, so one barrier per task suffices (a wave
 // writing buffer p again has passed the barrier that every reader of its
-// previous use reached after reading).  Task t owns 4 KB of partials: FP64
-// recursions store 2 x 256 doubles there, FP32 recursions (whose per-wave
-// sums are FP32) 2 x 256 floats in its first half, which halves their traffic.
-template <typename T, int V>
-__device__ __forceinline__ void flush_task(const T (&sxy)[V], const T (&syy)[V], int wave, int lane,
-                                           double (*sh)[2][256], void* __restrict__ part, size_t row2,
+// previous use reached after reading).  Task t owns 4 KB of partials: 2 x 256
+// doubles (FP64 in both recursion precisions).
+template <int V>
+__device__ __forceinline__ void flush_task(const double (&sxy)[V], const double (&syy)[V], int wave, int lane,
+                                           double (*sh)[2][256], double* __restrict__ part, size_t row2,
                                            int pcol) {
   constexpr int W = 64 * V;
 #pragma unroll
   for (int v = 0; v < V; v++) {
-    sh[wave][0][lane * V + v] = (double)sxy[v];
-    sh[wave][1][lane * V + v] = (double)syy[v];
+    sh[wave][0][lane * V + v] = sxy[v];
+    sh[wave][1][lane * V + v] = syy[v];
   }
   __syncthreads();
   for (int e = threadIdx.x; e < 2 * W; e += 256) {
     const int which = e / W, cc = e - which * W;
     const double s = (sh[0][which][cc] + sh[1][which][cc]) + (sh[2][which][cc] + sh[3][which][cc]);
-    if constexpr (sizeof(T) == 4)
-      static_cast<float*>(part)[row2 * 512 + which * 256 + pcol + cc] = (float)s;
-    else
-      static_cast<double*>(part)[(row2 + which) * 256 + pcol + cc] = s;
+    part[(row2 + which) * 256 + pcol + cc] = s;
   }
 }
 
@@ -102,14 +98,14 @@ template <typename T, int V, int MODE, bool NT>
 __device__ __forceinline__ void task_pass(const int* __restrict__ rowptr, const int* __restrict__ col,
                                           const T* __restrict__ hv, const T* xc, T* xn, int a0, int ab,
                                           int ae, int ld, int c0, int nvalid, int wave, int lane,
-                                          double (*sh)[2][256], void* __restrict__ part, size_t row2,
+                                          double (*sh)[2][256], double* __restrict__ part, size_t row2,
                                           int pcol) {
-  T sxy[V], syy[V];
+  double sxy[V], syy[V];
 #pragma unroll
-  for (int v = 0; v < V; v++) sxy[v] = syy[v] = T(0);
+  for (int v = 0; v < V; v++) sxy[v] = syy[v] = 0.0;
   for (int A = ab; A < ae; A += 4)
     ltbdev::atom_step<T, V, MODE, NT>(rowptr, col, hv, xc, xn, a0 + A, ld, 0, c0, nvalid, sxy, syy, a0);
-  flush_task<T, V>(sxy, syy, wave, lane, sh, part, row2, pcol);
+  flush_task<V>(sxy, syy, wave, lane, sh, part, row2, pcol);
 }
 
 template <typename T, int MODE, bool NT>
@@ -117,7 +113,7 @@ __global__ __launch_bounds__(256) void k_batch_step(const int* __restrict__ rowp
                                                     const T* __restrict__ hv, const char* __restrict__ xcur,
                                                     char* __restrict__ xnext, int nslots,
                                                     const BatchTask* __restrict__ tasks,
-                                                    void* __restrict__ part, int apw) {
+                                                    double* __restrict__ part, int apw) {
   constexpr int want = sizeof(T) == 4 ? 1 : 2;
   __shared__ double sh[2][4][2][256];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -165,7 +161,7 @@ __global__ __launch_bounds__(256) void k_batch_step(const int* __restrict__ rowp
 // per column of every active structure: fold the task partials of step n into
 // per-atom moments (as k_cheb_reduce for one structure)
 __global__ __launch_bounds__(256) void k_batch_reduce(
-    const void* __restrict__ part, int ncols, const int* __restrict__ gc_s,
+    const double* __restrict__ part, int ncols, const int* __restrict__ gc_s,
     const int* __restrict__ gc_c, const int* __restrict__ tbegin, const int* __restrict__ nsg,
     const int* __restrict__ aoff, const int* __restrict__ active, int n, int M,
     double* __restrict__ mu1, double* __restrict__ mom, int nat_total) {
@@ -180,9 +176,7 @@ __global__ __launch_bounds__(256) void k_batch_reduce(
   }
   double sxy = 0, syy = 0;
   bool valid = false;
-  int act = 0;
   if (use) {
-    act = active[s];
     const int nat = aoff[s + 1] - aoff[s];
     valid = c < 4 * nat;
     // 256-wide groups first, then the one tail group
@@ -190,18 +184,9 @@ __global__ __launch_bounds__(256) void k_batch_reduce(
     const int gidx = c < cw ? c >> 8 : cw >> 8;
     const int off = c < cw ? c & 255 : c - cw;
     const int base = tbegin[s] + gidx * nsg[s];
-    if (act == 1) {  // FP32 recursion: float partials (flush_task)
-      const float* pf = static_cast<const float*>(part);
-      for (int g = 0; g < nsg[s]; g++) {
-        sxy += (double)pf[(size_t)(base + g) * 1024 + off];
-        syy += (double)pf[(size_t)(base + g) * 1024 + 256 + off];
-      }
-    } else {
-      const double* pd = static_cast<const double*>(part);
-      for (int g = 0; g < nsg[s]; g++) {
-        sxy += pd[(size_t)(base + g) * 512 + off];
-        syy += pd[(size_t)(base + g) * 512 + 256 + off];
-      }
+    for (int g = 0; g < nsg[s]; g++) {
+      sxy += part[(size_t)(base + g) * 512 + off];
+      syy += part[(size_t)(base + g) * 512 + 256 + off];
     }
   }
   double modd = 0, meven = 0;

@@ -129,12 +129,12 @@ __global__ __launch_bounds__(256) void k_cheb_blocked(const int* __restrict__ ro
   const int c0 = cg * W + lane * V;
   const int a_begin = (sg * 4 + wave) * apw;
   const int a_end = min(a_begin + apw, nat);
-  T sxy[V], syy[V];
+  double sxy[V], syy[V];
 #pragma unroll
-  for (int v = 0; v < V; v++) sxy[v] = syy[v] = T(0);
+  for (int v = 0; v < V; v++) sxy[v] = syy[v] = 0.0;
   for (int A = a_begin; A < a_end; ++A)
     atom_step<T, V, MODE, NT>(rowptr, col, hv, xcur, xnext, A, ld, j0, c0, nvalid, sxy, syy);
-  flush_partials<T, V>(sxy, syy, wave, lane, part, (size_t)sg * 2, ld, cg * W);
+  flush_partials<V>(sxy, syy, wave, lane, part, (size_t)sg * 2, ld, cg * W);
 }
 
 template <typename T, int V, int MODE, bool NT>
@@ -158,13 +158,13 @@ __global__ __launch_bounds__(256) void k_cheb_sweep(const int* __restrict__ rowp
   const int wid = 4 * slot + wave;
   for (int cg = xcd; cg < n_cg; cg += 8) {
     const int c0 = cg * W + lane * V;
-    T sxy[V], syy[V];
+    double sxy[V], syy[V];
 #pragma unroll
-    for (int v = 0; v < V; v++) sxy[v] = syy[v] = T(0);
+    for (int v = 0; v < V; v++) sxy[v] = syy[v] = 0.0;
     for (int i = wid; i < nat; i += S)
       atom_step<T, V, MODE, NT>(rowptr, col, hv, xcur, xnext, visit ? visit[i] : i, ld, j0, c0, nvalid,
                                 sxy, syy);
-    flush_partials<T, V>(sxy, syy, wave, lane, part, (size_t)(xcd * slots + slot) * 2, ld, cg * W);
+    flush_partials<V>(sxy, syy, wave, lane, part, (size_t)(xcd * slots + slot) * 2, ld, cg * W);
   }
 }
 

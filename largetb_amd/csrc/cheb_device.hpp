@@ -45,8 +45,8 @@ template <typename T, int V, int MODE, bool NT>
 __device__ __forceinline__ void atom_step(const int* __restrict__ rowptr,
                                           const int* __restrict__ col, const T* __restrict__ hv,
                                           const T* __restrict__ xcur, T* __restrict__ xnext, int A,
-                                          int ld, int j0, int c0, int nvalid, T (&sxy)[V],
-                                          T (&syy)[V], int colbase = 0) {
+                                          int ld, int j0, int c0, int nvalid, double (&sxy)[V],
+                                          double (&syy)[V], int colbase = 0) {
   constexpr int G = 4;
   const int kb = rowptr[A], ke = rowptr[A + 1];
   // A indexes rowptr (global block row); rows of the vectors are local: A - colbase
@@ -116,26 +116,29 @@ __device__ __forceinline__ void atom_step(const int* __restrict__ rowptr,
     }
 #pragma unroll
   for (int r = 0; r < 4; r++) store_row<T, V, NT>(out + (size_t)r * ld, y[r]);
+  // the per-column dots are accumulated in FP64 (a product of two floats is
+  // exact in double, so only the FP64 sums round)
 #pragma unroll
   for (int v = 0; v < V; v++)
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      sxy[v] = fma(y[r][v], xa[r][v], sxy[v]);
-      syy[v] = fma(y[r][v], y[r][v], syy[v]);
+      const double yd = (double)y[r][v];
+      sxy[v] = fma(yd, (double)xa[r][v], sxy[v]);
+      syy[v] = fma(yd, yd, syy[v]);
     }
 }
 
 // sum the 4 waves' per-column partials in LDS (FP64) and write one row pair
-template <typename T, int V>
-__device__ __forceinline__ void flush_partials(const T (&sxy)[V], const T (&syy)[V], int wave,
+template <int V>
+__device__ __forceinline__ void flush_partials(const double (&sxy)[V], const double (&syy)[V], int wave,
                                                int lane, double* __restrict__ part, size_t row2,
                                                int ld, int colbase) {
   constexpr int W = 64 * V;
   __shared__ double sh[4][2][W];
 #pragma unroll
   for (int v = 0; v < V; v++) {
-    sh[wave][0][lane * V + v] = (double)sxy[v];
-    sh[wave][1][lane * V + v] = (double)syy[v];
+    sh[wave][0][lane * V + v] = sxy[v];
+    sh[wave][1][lane * V + v] = syy[v];
   }
   __syncthreads();
   for (int e = threadIdx.x; e < 2 * W; e += 256) {

@@ -895,6 +895,19 @@ int ltb_density_diag(ltb_ctx* c, double* diagP) {
   return LTB_OK;
 }
 
+int ltb_get_orbital_moments(ltb_ctx* c, int M, int j0, int j1, double* out) {
+  int st = need_system(c);
+  if (st) return st;
+  if (!out || j0 < 0 || j1 > c->norb || j0 >= j1) return set_err(c, LTB_ERR_ARG, "bad orbital range");
+  if (c->momorb_M < 0 || c->momorb_M != M)
+    return set_err(c, LTB_ERR_STATE, "needs ltb_set_kpe_orbital_moments(1) and ltb_kpe_moments with this M");
+  const size_t w = sizeof(double) * (size_t)(j1 - j0);
+  LTB_HIP(c, hipMemcpy2DAsync(out, w, c->d_momorb + j0, sizeof(double) * (size_t)c->norb, w, (size_t)M + 1,
+                              hipMemcpyDeviceToHost, c->stream));
+  LTB_HIP(c, hipStreamSynchronize(c->stream));
+  return LTB_OK;
+}
+
 int ltb_kpe_weights(ltb_ctx* c, int M, double* w) {
   int st = need_system(c);
   if (st) return st;

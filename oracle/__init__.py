@@ -81,6 +81,7 @@ def _setup(L):
     L.orc_fd_coeffs.restype = None
     L.orc_rescaled_csr.argtypes = [C.c_int, PI, PI, PD, PD, C.c_double, C.c_double, PI, PI, PD]
     L.orc_kpe_diag_moments.argtypes = [C.c_int, PI, PI, PD, C.c_int, C.c_int, C.c_int, PD, C.c_int]
+    L.orc_kpe_diag_moments_cols.argtypes = [C.c_int, PI, PI, PD, C.c_int, C.c_int, PI, PD, C.c_int]
     L.orc_kpe_apply.argtypes = [C.c_int, PI, PI, PD, C.c_int, PD, PD, PD]
     L.orc_count.argtypes = [C.c_int, PD, PD]
     L.orc_count.restype = C.c_double
@@ -206,6 +207,16 @@ class System:
                                           _p(cval, C.c_double), nm, j0, j1, _p(out, C.c_double),
                                           nthreads), "orc_kpe_diag_moments")
         return out.reshape(j1 - j0, nm)
+
+    def diag_moments_at(self, a, b, nm, cols, shift=None, nthreads=0):
+        """[T_n]_jj, n < nm, for the listed orbitals j: [len(cols), nm]"""
+        cols = np.ascontiguousarray(cols, np.int32)
+        crow, ccol, cval = self.rescaled_csr(a, b, shift)
+        out = np.zeros(len(cols) * nm)
+        _check(lib().orc_kpe_diag_moments_cols(self.norb, _p(crow, C.c_int), _p(ccol, C.c_int),
+                                               _p(cval, C.c_double), nm, len(cols), _p(cols, C.c_int),
+                                               _p(out, C.c_double), nthreads), "orc_kpe_diag_moments_cols")
+        return out.reshape(len(cols), nm)
 
     def kpe_apply(self, a, b, w, x, shift=None):
         crow, ccol, cval = self.rescaled_csr(a, b, shift)

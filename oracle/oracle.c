@@ -428,10 +428,31 @@ static void vaxpy(int n, double *y, double a, const double *x) {
   for (int i = 0; i < n; i++) y[i] += a * x[i];
 }
 
-/* getKPE's recursion (deltaH:1139-1168) on e_j, reading [T_n]_jj */
-int orc_kpe_diag_moments(int n, const int *crow, const int *ccol, const double *cval, int nm,
-                         int j0, int j1, double *mom, int nthreads) {
-  if (nm < 2 || j0 < 0 || j1 > n || j0 > j1) return ORC_ERR_ARG;
+/* getKPE's recursion (deltaH:1139-1168) on e_j, reading [T_n]_jj into
+ * out[0..nm); xi, xm1, xp1, hx are n-vectors of scratch */
+static void diag_moments_one(int n, const int *crow, const int *ccol, const double *cval, int nm, int j,
+                             double *out, double *xi, double *xm1, double *xp1, double *hx) {
+  memset(xi, 0, sizeof(double) * n);
+  xi[j] = 1.0;                /* VecCopy(randomVector, x_i)         */
+  out[0] = xi[j];
+  spmv(n, crow, ccol, cval, xi, hx); /* MatMult(Htotal, x_i, Hx_i)  */
+  vcopy(n, xi, xm1);                 /* VecCopy(x_i, x_m1)          */
+  vcopy(n, hx, xi);                  /* VecCopy(Hx_i, x_i)          */
+  out[1] = xi[j];
+  for (int m = 2; m < nm; m++) {
+    spmv(n, crow, ccol, cval, xi, hx); /* MatMult                  */
+    vcopy(n, hx, xp1);                 /* VecCopy(Hx_i, x_p1)       */
+    vscale(n, xp1, 2.0);               /* VecScale(x_p1, 2.0)       */
+    vaxpy(n, xp1, -1.0, xm1);          /* VecAXPY(x_p1, -1, x_m1)   */
+    vcopy(n, xi, xm1);                 /* VecCopy(x_i, x_m1)        */
+    vcopy(n, xp1, xi);                 /* VecCopy(x_p1, x_i)        */
+    out[m] = xi[j];                    /* theta-AXPY -> moment read */
+  }
+}
+
+/* the columns cols[0..ncols) (or j0 + k when cols is NULL) */
+static int diag_moments_list(int n, const int *crow, const int *ccol, const double *cval, int nm, int ncols,
+                             int j0, const int *cols, double *mom, int nthreads) {
 #ifdef _OPENMP
   if (nthreads <= 0) nthreads = omp_get_max_threads();
 #else
@@ -444,31 +465,29 @@ int orc_kpe_diag_moments(int n, const int *crow, const int *ccol, const double *
     double *xp1 = (double *)malloc(sizeof(double) * n);
     double *hx = (double *)malloc(sizeof(double) * n);
 #pragma omp for schedule(dynamic, 1)
-    for (int j = j0; j < j1; j++) {
-      double *out = mom + (size_t)(j - j0) * nm;
-      memset(xi, 0, sizeof(double) * n);
-      xi[j] = 1.0;                /* VecCopy(randomVector, x_i)         */
-      out[0] = xi[j];
-      spmv(n, crow, ccol, cval, xi, hx); /* MatMult(Htotal, x_i, Hx_i)  */
-      vcopy(n, xi, xm1);                 /* VecCopy(x_i, x_m1)          */
-      vcopy(n, hx, xi);                  /* VecCopy(Hx_i, x_i)          */
-      out[1] = xi[j];
-      for (int m = 2; m < nm; m++) {
-        spmv(n, crow, ccol, cval, xi, hx); /* MatMult                  */
-        vcopy(n, hx, xp1);                 /* VecCopy(Hx_i, x_p1)       */
-        vscale(n, xp1, 2.0);               /* VecScale(x_p1, 2.0)       */
-        vaxpy(n, xp1, -1.0, xm1);          /* VecAXPY(x_p1, -1, x_m1)   */
-        vcopy(n, xi, xm1);                 /* VecCopy(x_i, x_m1)        */
-        vcopy(n, xp1, xi);                 /* VecCopy(x_p1, x_i)        */
-        out[m] = xi[j];                    /* theta-AXPY -> moment read */
-      }
-    }
+    for (int k = 0; k < ncols; k++)
+      diag_moments_one(n, crow, ccol, cval, nm, cols ? cols[k] : j0 + k, mom + (size_t)k * nm, xi, xm1, xp1,
+                       hx);
     free(xi);
     free(xm1);
     free(xp1);
     free(hx);
   }
   return ORC_OK;
+}
+
+int orc_kpe_diag_moments(int n, const int *crow, const int *ccol, const double *cval, int nm,
+                         int j0, int j1, double *mom, int nthreads) {
+  if (nm < 2 || j0 < 0 || j1 > n || j0 > j1) return ORC_ERR_ARG;
+  return diag_moments_list(n, crow, ccol, cval, nm, j1 - j0, j0, NULL, mom, nthreads);
+}
+
+int orc_kpe_diag_moments_cols(int n, const int *crow, const int *ccol, const double *cval, int nm,
+                              int ncols, const int *cols, double *mom, int nthreads) {
+  if (nm < 2 || ncols < 0 || !cols) return ORC_ERR_ARG;
+  for (int k = 0; k < ncols; k++)
+    if (cols[k] < 0 || cols[k] >= n) return ORC_ERR_ARG;
+  return diag_moments_list(n, crow, ccol, cval, nm, ncols, 0, cols, mom, nthreads);
 }
 
 /* getKPE deltaH:1068-1185 with coefficients w */

@@ -84,6 +84,9 @@ int orc_rescaled_csr(int nat, const int *rowptr, const int *col, const double *v
  * n < nm.  nthreads<=0 -> OpenMP default. */
 int orc_kpe_diag_moments(int n, const int *crow, const int *ccol, const double *cval, int nm,
                          int j0, int j1, double *mom_orb, int nthreads);
+/* the same for the listed columns: mom_orb[k*nm + n] = [T_n]_{cols[k] cols[k]} */
+int orc_kpe_diag_moments_cols(int n, const int *crow, const int *ccol, const double *cval, int nm,
+                              int ncols, const int *cols, double *mom_orb, int nthreads);
 
 /* getKPE (deltaH:1068-1185): out = w0*x + sum_{m>=1} w_m T_m x, with
  * w = (c0/2, c1 g1, ..., c_{M-1} g_{M-1}) given in w[M]. */
