@@ -293,6 +293,19 @@ int ltb_write_dos(const char* path, int n, const double* energies, const double*
  * w = (c_0/2, c_1 g_1, ...).  X, Y: [ncols][norb] column vectors (host). */
 int ltb_kpe_apply(ltb_ctx* ctx, int M, const double* w, int ncols, const double* X, double* Y,
                   int precision);
+/* Allocation-free form of ltb_kpe_apply.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  X and Y are DEVICE
+ * buffers of the caller, [ncols][norb] FP64 column vectors; w stays a host
+ * array.  The recursion vectors and theta live in one fixed 96 MiB scratch
+ * array per GPU (norb * ld * (2 * sizeof(T) + 8) bytes, ld = ncols rounded up
+ * to 64, T the recursion precision); larger calls fail with LTB_ERR_ARG and a
+ * message naming the largest column count that fits.  It calls no hipMalloc
+ * or hipFree.  Calls on one GPU share the array, so calls that overlap in
+ * time on one GPU (other contexts, other threads) race; each call clears what
+ * it reads before reading it.  Returns after the context stream has drained. */
+int ltb_kpe_apply_noalloc(ltb_ctx* ctx, int M, const double* w, int ncols, const double* X, double* Y,
+                          int precision);
 
 /* SP2 (getSP2 :1187-1215 / computeAlphas :1217-1294) on an explicit dense
  * matrix: X0 = (emax I - H_total)/(emax - emin) from the Gershgorin bounds,

@@ -118,6 +118,7 @@ SIGNATURES = {
     "ltb_density": (C.c_int, [VP, C.POINTER(FermiParams), PD, C.POINTER(DensityResult)]),
     "ltb_scc_run": (C.c_int, [VP, C.POINTER(SccParams), PD, PD, PD, C.POINTER(SccResult)]),
     "ltb_kpe_apply": (C.c_int, [VP, C.c_int, PD, C.c_int, PD, PD, C.c_int]),
+    "ltb_kpe_apply_noalloc": (C.c_int, [VP, C.c_int, PD, C.c_int, VP, VP, C.c_int]),
     "ltb_kpe_block_moments": (C.c_int, [VP, C.c_int, C.c_int, PD, C.c_int, PD, PD]),
     "ltb_random_vectors": (C.c_int, [C.c_ulonglong, C.c_int, C.c_int, PD]),
     "ltb_kpe_stochastic": (C.c_int, [VP, C.c_int, C.c_int, C.c_ulonglong, C.c_int, PD]),

@@ -69,6 +69,9 @@ struct ltb_ctx {
   // SP2 dense buffers
   void *d_sa = nullptr, *d_sb = nullptr;
   size_t sbytes = 0;
+  // device address of the allocation-free entry point's scratch array on this
+  // context's device (not owned: the array lives as long as the process)
+  void* d_scratch = nullptr;
 };
 
 int set_err(ltb_ctx* c, int code, const std::string& msg);
@@ -108,5 +111,6 @@ int kpe_apply_device(ltb_ctx* c, int M, const double* w, int ld, void* a, void* 
 int kpe_time_steps(ltb_ctx* c, int precision, int steps, double* seconds, double* bytes);
 int kpe_resolved_tiling(ltb_ctx* c, int precision, ltb_kpe_tiling* out);
 typedef ltb_kpe_tiling KpeTiling;
+int noalloc_scratch_init(ltb_ctx* c);
 int sp2_dense_impl(ltb_ctx* c, double Ne, int max_iter, int fp64_tail, double* diagP, double* pop,
                    ltb_sp2_result* res);

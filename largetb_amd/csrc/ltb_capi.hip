@@ -567,6 +567,13 @@ int ltb_create(int device, ltb_ctx** out) {
     g_sys_err = "device scalar initialisation failed";
     return LTB_ERR_DEVICE;
   }
+  if (noalloc_scratch_init(c) != LTB_OK) {
+    g_sys_err = c->err;
+    free_all(c);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return LTB_ERR_DEVICE;
+  }
   *out = c;
   return LTB_OK;
 }
