@@ -6,6 +6,7 @@ the library is missing: there is no CPU fallback for the device calls.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -185,6 +186,14 @@ def _d(a):
     return a.ctypes.data_as(PD)
 
 
+def _vec(a, n, what):
+    """a as a contiguous float64 vector of length n (the C side reads n doubles)"""
+    v = np.ascontiguousarray(a, np.float64).reshape(-1)
+    if v.size != n:
+        raise ValueError(f"{what}: {v.size} values given, {n} expected")
+    return v
+
+
 def _i(a):
     return a.ctypes.data_as(PI)
 
@@ -345,11 +354,21 @@ class Context:
             raise LtbError(st, lib().ltb_last_error(None).decode())
         self._h = h
         self.system = None
+        self._children = weakref.WeakSet()  # Batch / Mlp objects holding this context
 
     def close(self):
+        """Destroy the context; its Batch and Mlp children are closed first
+        (their device objects refer to the context's stream)."""
         if getattr(self, "_h", None) is not None:
+            for child in list(self._children):
+                child.close()
             lib().ltb_destroy(self._h)
             self._h = None
+
+    def _handle(self):
+        if getattr(self, "_h", None) is None:
+            raise LtbError(LTB_ERR_STATE, "context is closed")
+        return self._h
 
     def __del__(self):
         if _lib is not None:
@@ -409,8 +428,8 @@ class Context:
         """Coulomb shift from per-spin populations p; None keeps the
         populations the context holds (neutral after load, the last mixed
         ones after an SCC)."""
-        pp = _d(np.ascontiguousarray(p, np.float64)) if p is not None else None
-        self._check(lib().ltb_set_populations(self._h, pp))
+        pv = _vec(p, self.nat, "populations") if p is not None else None
+        self._check(lib().ltb_set_populations(self._h, _d(pv) if pv is not None else None))
 
     def shift(self):
         v = np.zeros(self.nat)
@@ -418,7 +437,8 @@ class Context:
         return v
 
     def set_neutral(self, p0):
-        self._check(lib().ltb_set_neutral(self._h, _d(np.ascontiguousarray(p0, np.float64))))
+        p0 = _vec(p0, self.nat, "neutral populations")
+        self._check(lib().ltb_set_neutral(self._h, _d(p0)))
 
     def mixed_populations(self):
         """Per-spin populations the next SCC iteration starts from."""
@@ -456,8 +476,9 @@ class Context:
         pout = np.zeros(self.nat)
         hist = np.zeros(max_iter)
         r = SccResult()
-        g = _d(np.ascontiguousarray(guess, np.float64)) if guess is not None else None
-        st = lib().ltb_scc_run(self._h, C.byref(prm), g, _d(pout), _d(hist), C.byref(r))
+        gv = _vec(guess, self.nat, "guess") if guess is not None else None
+        st = lib().ltb_scc_run(self._h, C.byref(prm), _d(gv) if gv is not None else None, _d(pout), _d(hist),
+                               C.byref(r))
         if check and st not in (LTB_OK,):
             self._check(st)
         return dict(status=st, pop=pout, q=2 * pout, iters=r.iterations, fp64_iters=r.fp64_iterations,
@@ -570,8 +591,9 @@ class Batch:
         self.ctx = ctx
         arr = (VP * len(systems))(*[s._h for s in systems])
         h = VP()
-        ctx._check(lib().ltb_batch_create(ctx._h, len(systems), arr, C.byref(h)))
+        ctx._check(lib().ltb_batch_create(ctx._handle(), len(systems), arr, C.byref(h)))
         self._h = h
+        ctx._children.add(self)
         S, nat = C.c_int(), C.c_int()
         off = np.zeros(len(systems) + 1, np.int32)
         ctx._check(lib().ltb_batch_sizes(h, C.byref(S), C.byref(nat), _i(off)))
@@ -602,9 +624,9 @@ class Batch:
         mu = np.zeros(self.S)
         status = np.zeros(self.S, np.int32)
         r = SccResult()
-        g = _d(np.ascontiguousarray(guess, np.float64)) if guess is not None else None
-        st = lib().ltb_batch_scc_run(self._h, C.byref(prm), g, _d(pout), _i(its), _d(ebfp), _d(mu),
-                                     _i(status), C.byref(r))
+        gv = _vec(guess, self.nat, "guess") if guess is not None else None
+        st = lib().ltb_batch_scc_run(self._batch(), C.byref(prm), _d(gv) if gv is not None else None, _d(pout),
+                                     _i(its), _d(ebfp), _d(mu), _i(status), C.byref(r))
         if check:
             self.ctx._check(st)
         return dict(status=st, pop=pout, q=2 * pout, iters=its, eband_fp=ebfp, mu_ha=mu,
@@ -613,22 +635,27 @@ class Batch:
 
     def tiling(self):
         t = KpeTiling()
-        self.ctx._check(lib().ltb_batch_get_tiling(self._h, C.byref(t)))
+        self.ctx._check(lib().ltb_batch_get_tiling(self._batch(), C.byref(t)))
         return {k: getattr(t, k) for k, _ in KpeTiling._fields_}
 
     def set_tiling(self, atoms_per_wave=0, blocks_per_cu=0, nontemporal=-1):
         t = KpeTiling(0, 0, 0, blocks_per_cu, atoms_per_wave, nontemporal, 0)
-        self.ctx._check(lib().ltb_batch_set_tiling(self._h, C.byref(t)))
+        self.ctx._check(lib().ltb_batch_set_tiling(self._batch(), C.byref(t)))
 
     def autotune(self, precision=FP32_ACC64, trial_steps=3):
         t, g = KpeTiling(), C.c_double()
-        self.ctx._check(lib().ltb_batch_autotune(self._h, precision, trial_steps, C.byref(t), C.byref(g)))
+        self.ctx._check(lib().ltb_batch_autotune(self._batch(), precision, trial_steps, C.byref(t), C.byref(g)))
         return {k: getattr(t, k) for k, _ in KpeTiling._fields_}, g.value
 
     def stats(self):
         n, sec, by = C.c_longlong(), C.c_double(), C.c_double()
-        self.ctx._check(lib().ltb_batch_stats(self._h, C.byref(n), C.byref(sec), C.byref(by)))
+        self.ctx._check(lib().ltb_batch_stats(self._batch(), C.byref(n), C.byref(sec), C.byref(by)))
         return dict(launches=n.value, seconds=sec.value, bytes=by.value)
+
+    def _batch(self):
+        if getattr(self, "_h", None) is None:
+            raise LtbError(LTB_ERR_STATE, "batch is closed")
+        return self._h
 
     def split(self, x):
         """Per-structure views of a concatenated per-atom array."""
@@ -641,8 +668,9 @@ class Mlp:
     def __init__(self, ctx, seed=1):
         self.ctx = ctx
         h = VP()
-        ctx._check(lib().ltb_mlp_create(ctx._h, seed, C.byref(h)))
+        ctx._check(lib().ltb_mlp_create(ctx._handle(), seed, C.byref(h)))
         self._h = h
+        ctx._children.add(self)
         self.nparams = lib().ltb_mlp_num_params()
 
     def close(self):
@@ -650,50 +678,60 @@ class Mlp:
             lib().ltb_mlp_destroy(self._h)
             self._h = None
 
+    def _mlp(self):
+        if getattr(self, "_h", None) is None:
+            raise LtbError(LTB_ERR_STATE, "network is closed")
+        return self._h
+
     def __del__(self):
         if _lib is not None:
             self.close()
 
     def params(self):
         p = np.zeros(self.nparams, np.float32)
-        self.ctx._check(lib().ltb_mlp_get_params(self._h, _f(p)))
+        self.ctx._check(lib().ltb_mlp_get_params(self._mlp(), _f(p)))
         return p
 
     def set_params(self, p):
         p = np.ascontiguousarray(p, np.float32)
         assert p.size == self.nparams
-        self.ctx._check(lib().ltb_mlp_set_params(self._h, _f(p)))
+        self.ctx._check(lib().ltb_mlp_set_params(self._mlp(), _f(p)))
 
     def set_data(self, feat, Z, seg, qtot, target=None):
+        Z = np.ascontiguousarray(Z, np.int32).reshape(-1)
+        n = len(Z)
         feat = np.ascontiguousarray(feat, np.float32)
-        Z = np.ascontiguousarray(Z, np.int32)
-        seg = np.ascontiguousarray(seg, np.int32)
-        qtot = np.ascontiguousarray(qtot, np.float64)
-        self._target = None if target is None else np.ascontiguousarray(target, np.float64)
+        if feat.shape != (n, MLP_NFEAT):
+            raise ValueError(f"features: shape {feat.shape}, ({n}, {MLP_NFEAT}) expected")
+        seg = np.ascontiguousarray(seg, np.int32).reshape(-1)
+        if len(seg) < 2:
+            raise ValueError("segments: at least one structure [seg[0], seg[1]) expected")
+        qtot = _vec(qtot, len(seg) - 1, "electrons per structure")
+        self._target = None if target is None else _vec(target, n, "target charges")
         self.n = len(Z)
-        self.ctx._check(lib().ltb_mlp_set_data(self._h, len(Z), _f(feat), _i(Z), len(seg) - 1, _i(seg),
+        self.ctx._check(lib().ltb_mlp_set_data(self._mlp(), len(Z), _f(feat), _i(Z), len(seg) - 1, _i(seg),
                                                _d(qtot), _d(self._target) if self._target is not None else None))
 
     def fit_normalization(self):
-        self.ctx._check(lib().ltb_mlp_fit_normalization(self._h))
+        self.ctx._check(lib().ltb_mlp_fit_normalization(self._mlp()))
 
     def forward(self):
         q = np.zeros(self.n)
-        self.ctx._check(lib().ltb_mlp_forward(self._h, _d(q)))
+        self.ctx._check(lib().ltb_mlp_forward(self._mlp(), _d(q)))
         return q
 
     def loss_grad(self):
         loss = C.c_double()
         g = np.zeros(MLP_TRAINABLE, np.float32)
-        self.ctx._check(lib().ltb_mlp_loss_grad(self._h, C.byref(loss), _f(g)))
+        self.ctx._check(lib().ltb_mlp_loss_grad(self._mlp(), C.byref(loss), _f(g)))
         return loss.value, g
 
     def adam(self, steps, lr=1e-3):
         hist = np.zeros(steps)
-        self.ctx._check(lib().ltb_mlp_adam(self._h, steps, lr, _d(hist)))
+        self.ctx._check(lib().ltb_mlp_adam(self._mlp(), steps, lr, _d(hist)))
         return hist
 
     def predict(self, system):
         p = np.zeros(system.nat)
-        self.ctx._check(lib().ltb_mlp_predict(self._h, system._h, _d(p)))
+        self.ctx._check(lib().ltb_mlp_predict(self._mlp(), system._h, _d(p)))
         return p

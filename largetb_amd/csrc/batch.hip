@@ -1030,42 +1030,48 @@ int ltb_batch_autotune(ltb_batch* b, int precision, int trial_steps, ltb_kpe_til
   const ltb_kpe_tiling keep = b->tiling;
   ltb_kpe_tiling top = keep;
   double top_gbs = -1;
+  // one flat list of candidates, so the first failure leaves the search
+  struct Cand {
+    int apw, bpc, nt;
+  };
+  std::vector<Cand> cands;
   for (int apw : {4, 8, 16})
     for (int bpc : {2, 4, 8})
-      for (int nt : {0, 1}) {
-        b->tiling.atoms_per_wave = apw;
-        b->tiling.blocks_per_cu = bpc;
-        b->tiling.nontemporal = nt;
-        if ((st = build_tasks(b))) break;
-        const int grid = step_grid(b);
-        auto step = [&](int n) {
-          const int mode = n == 0 ? 0 : (n == 1 ? 1 : 2);
-          void *cur, *nxt;
-          if (precision == LTB_FP64) {
-            step_buffers<double>(b, n, &cur, &nxt);
-            launch_step<double>(b, mode, grid, cur, nxt);
-          } else {
-            step_buffers<float>(b, n, &cur, &nxt);
-            launch_step<float>(b, mode, grid, cur, nxt);
-          }
-        };
-        step(0);
-        step(1);
-        (void)hipEventRecord(e0, c->stream);
-        for (int n = 2; n < 2 + trial_steps; n++) step(n);
-        (void)hipEventRecord(e1, c->stream);
-        if (hipStreamSynchronize(c->stream) != hipSuccess || hipGetLastError() != hipSuccess) {
-          st = set_err(c, LTB_ERR_DEVICE, "batch autotune trial failed");
-          break;
-        }
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        const double gbs = step_bytes * trial_steps / (ms * 1e-3) / 1e9;
-        if (gbs > top_gbs) {
-          top_gbs = gbs;
-          top = b->tiling;
-        }
+      for (int nt : {0, 1}) cands.push_back({apw, bpc, nt});
+  for (const Cand& cd : cands) {
+    b->tiling.atoms_per_wave = cd.apw;
+    b->tiling.blocks_per_cu = cd.bpc;
+    b->tiling.nontemporal = cd.nt;
+    if ((st = build_tasks(b))) break;
+    const int grid = step_grid(b);
+    auto step = [&](int n) {
+      const int mode = n == 0 ? 0 : (n == 1 ? 1 : 2);
+      void *cur, *nxt;
+      if (precision == LTB_FP64) {
+        step_buffers<double>(b, n, &cur, &nxt);
+        launch_step<double>(b, mode, grid, cur, nxt);
+      } else {
+        step_buffers<float>(b, n, &cur, &nxt);
+        launch_step<float>(b, mode, grid, cur, nxt);
       }
+    };
+    step(0);
+    step(1);
+    (void)hipEventRecord(e0, c->stream);
+    for (int n = 2; n < 2 + trial_steps; n++) step(n);
+    (void)hipEventRecord(e1, c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess || hipGetLastError() != hipSuccess) {
+      st = set_err(c, LTB_ERR_DEVICE, "batch autotune trial failed");
+      break;
+    }
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    const double gbs = step_bytes * trial_steps / (ms * 1e-3) / 1e9;
+    if (gbs > top_gbs) {
+      top_gbs = gbs;
+      top = b->tiling;
+    }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (st) {

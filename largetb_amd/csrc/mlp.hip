@@ -319,6 +319,17 @@ void free_data(ltb_mlp* m) {
   m->n = m->nseg = 0;
 }
 
+int upload_data(ltb_mlp* m, int n, const float* feat, const int* Z, int nseg, const int* seg, const double* qtot,
+                const double* target, const std::vector<int> (&idx)[2]);
+
+void free_params(ltb_mlp* m) {
+  void* ps[] = {m->d_P, m->d_grad, m->d_m, m->d_v, m->d_partial, m->d_loss};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  m->d_P = m->d_grad = m->d_m = m->d_v = m->d_partial = nullptr;
+  m->d_loss = nullptr;
+}
+
 int descriptors_device(ltb_ctx* c, const ltb_system* s, float* d_feat) {
   std::vector<int> rp, nj;
   std::vector<double> nd;
@@ -420,13 +431,18 @@ int ltb_mlp_create(ltb_ctx* c, unsigned long long seed, ltb_mlp** out) {
       hipMalloc(&m->d_v, sizeof(float) * NTRAIN) != hipSuccess ||
       hipMalloc(&m->d_partial, sizeof(float) * NTRAIN * NPART) != hipSuccess ||
       hipMalloc(&m->d_loss, sizeof(double)) != hipSuccess) {
+    free_params(m);
     delete m;
     return set_err(c, LTB_ERR_NOMEM, "MLP allocation failed");
   }
-  (void)hipMemcpy(m->d_P, P.data(), sizeof(float) * NPARAM, hipMemcpyHostToDevice);
-  (void)hipMemset(m->d_m, 0, sizeof(float) * NTRAIN);
-  (void)hipMemset(m->d_v, 0, sizeof(float) * NTRAIN);
-  (void)hipMemset(m->d_partial, 0, sizeof(float) * NTRAIN * NPART);
+  if (hipMemcpy(m->d_P, P.data(), sizeof(float) * NPARAM, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(m->d_m, 0, sizeof(float) * NTRAIN) != hipSuccess ||
+      hipMemset(m->d_v, 0, sizeof(float) * NTRAIN) != hipSuccess ||
+      hipMemset(m->d_partial, 0, sizeof(float) * NTRAIN * NPART) != hipSuccess) {
+    free_params(m);
+    delete m;
+    return set_err(c, LTB_ERR_DEVICE, "MLP parameter initialisation failed");
+  }
   *out = m;
   return LTB_OK;
 }
@@ -435,12 +451,7 @@ void ltb_mlp_destroy(ltb_mlp* m) {
   if (!m) return;
   (void)hipStreamSynchronize(m->ctx->stream);
   free_data(m);
-  (void)hipFree(m->d_P);
-  (void)hipFree(m->d_grad);
-  (void)hipFree(m->d_m);
-  (void)hipFree(m->d_v);
-  (void)hipFree(m->d_partial);
-  (void)hipFree(m->d_loss);
+  free_params(m);
   delete m;
 }
 
@@ -485,8 +496,24 @@ int ltb_mlp_set_data(ltb_mlp* m, int n, const float* feat, const int* Z, int nse
     if (Z[a] != 14 && Z[a] != 8) return set_err(c, LTB_ERR_ARG, "Z must be 14 or 8");
     idx[Z[a] == 14 ? 0 : 1].push_back(a);
   }
+  // n and nseg are set only once every buffer exists; any failure frees what
+  // was allocated and leaves the object without data (n = 0)
+  const int st = upload_data(m, n, feat, Z, nseg, seg, qtot, target, idx);
+  if (st) {
+    free_data(m);
+    return st;
+  }
   m->n = n;
   m->nseg = nseg;
+  return LTB_OK;
+}
+
+}  // extern "C"
+
+namespace {
+int upload_data(ltb_mlp* m, int n, const float* feat, const int* Z, int nseg, const int* seg, const double* qtot,
+                const double* target, const std::vector<int> (&idx)[2]) {
+  ltb_ctx* c = m->ctx;
   const size_t act = sizeof(float) * NH * (size_t)n;
   LTB_HIP(c, hipMalloc(&m->d_feat, sizeof(float) * NF * (size_t)n));
   for (float** p : {&m->d_z1, &m->d_z2, &m->d_z3, &m->d_d1, &m->d_d2, &m->d_d3}) LTB_HIP(c, hipMalloc(p, act));
@@ -510,6 +537,9 @@ int ltb_mlp_set_data(ltb_mlp* m, int n, const float* feat, const int* Z, int nse
   if (target) LTB_HIP(c, hipMemcpy(m->d_t, target, sizeof(double) * n, hipMemcpyHostToDevice));
   return LTB_OK;
 }
+}  // namespace
+
+extern "C" {
 
 int ltb_mlp_fit_normalization(ltb_mlp* m) {
   if (!m || m->n <= 0 || !m->has_target) return LTB_ERR_ARG;

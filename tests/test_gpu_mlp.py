@@ -81,3 +81,24 @@ def test_trained_guess_cuts_iterations_and_keeps_the_answer(ctx):
     assert np.abs(2 * guess - neutral["q"]).max() < 0.02
     assert guided["iters"] < neutral["iters"] - 20
     assert np.abs(guided["q"] - neutral["q"]).max() < 1e-6
+
+
+def test_context_close_closes_its_batch_and_network(ctx):
+    """Closing a context first closes the Batch and Mlp objects made on it
+    (they hold its stream); later calls on them raise instead of touching
+    freed state.  Short per-atom vectors are refused before the C call."""
+    c = L.Context(0)
+    s = L.System.quartz(2, 2, 2, sigma_ang=0.05, seed=1)
+    b = L.Batch(c, [s])
+    m = L.Mlp(c)
+    c.close()
+    assert b._h is None and m._h is None
+    with pytest.raises(L.LtbError):
+        b.scc(64)
+    with pytest.raises(L.LtbError):
+        m.params()
+    ctx.load(s)
+    with pytest.raises(ValueError):
+        ctx.scc(64, guess=np.zeros(s.nat - 1))
+    with pytest.raises(ValueError):
+        ctx.set_populations(np.zeros(s.nat + 1))

@@ -52,3 +52,14 @@ def test_bench_traffic_comes_from_the_committed_counter_summary():
     assert bench.measured_traffic([12, 12, 12], 512, dict(t, cols_per_lane=2), "fp32") is None
     assert bench.measured_traffic([12, 12, 12], 512, t, "fp64") is None
 
+
+
+def test_host_vectors_are_checked_before_the_c_side_reads_them():
+    """The C entry points read natoms doubles from every per-atom array, so the
+    front end refuses a vector of another length before the call."""
+    v = L._vec([1, 2, 3], 3, "p")
+    assert v.dtype == np.float64 and v.flags["C_CONTIGUOUS"]
+    with pytest.raises(ValueError, match="2 values given, 3 expected"):
+        L._vec(np.zeros(2), 3, "p")
+    with pytest.raises(ValueError):
+        L._vec(np.zeros((2, 2)), 3, "p")
