@@ -9,6 +9,10 @@ everything else FP64), inputs resident in HBM.
 value = algorithmic Chebyshev block-SpMM bytes of the timed iterations divided
 by their wall time (GB/s).  roofline = the block-SpMM kernel's algorithmic
 bytes per launch over its mean launch time (HIP events on the context stream).
+time_to_solution (default on, --no-tts): plain FP64 SCC from the neutral
+start against the learned guess with FP32 bulk and FP64 finish, both to
+|dq| < 1e-6 e, within --time-budget seconds of the start (else it says it was
+skipped); one line per SCC iteration goes to stderr meanwhile.
 """
 import argparse
 import json
@@ -21,26 +25,41 @@ sys.path.insert(0, ROOT)
 
 METRIC = "SCC time-to-solution, 15.5k-atom SiO2: learned+mixed vs plain FP64; SpMM GB/s"
 HBM_PEAK_GBS = 8000.0
+T_START = time.perf_counter()
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(cells, sigma, seed, M, target_s=15.0):
+def host_cores():
+    """Threads for the CPU leg: the process's CPU share as OMP_NUM_THREADS
+    states it, else every core the process may run on."""
+    try:
+        n = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        n = 0
+    if n > 0:
+        return n
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(cells, sigma, seed, M, gpu_bytes_per_iter, target_s=12.0):
     """The oracle's op-for-op FP64 restatement of getKPE (CSR SpMV + BLAS-1 per
-    step) on a sample of unit-vector columns of the same structure."""
+    step) on a sample of unit-vector columns of the same structure.
+
+    value: the GPU line's algorithmic bytes of one SCC iteration divided by the
+    CPU's seconds per SCC iteration (the sample's time per column times all
+    columns), i.e. the same work unit as the GPU value."""
     import numpy as np
     import oracle as O
     o = O.System.quartz(*cells, sigma_ang=sigma, seed=seed)
     lo, hi = o.gershgorin()
     a, b = (hi - lo) / 2, (hi + lo) / 2
-    cores = os.cpu_count() or 1
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        pass
-    cores = min(cores, 16)
+    cores = host_cores()
     # grow the sample until it takes at least 0.6 target_s (or covers every column)
     ncol = cores
     while True:
@@ -52,14 +71,20 @@ def cpu_baseline(cells, sigma, seed, M, target_s=15.0):
         ncol = min(o.norb, max(2 * ncol, int(ncol * target_s / max(t, 1e-3)) // cores * cores))
     n = o.norb
     nnz = o.nnzb * 16
-    # algorithmic bytes of the CPU recursion: 3 FP64 vector streams + CSR per step
-    bytes_per_col = M * (3 * 8.0 * n + nnz * 12.0 + 4.0 * (n + 1))
-    gbs = bytes_per_col * ncol / t / 1e9
+    # the oracle's own FP64 traffic: 3 vector streams + CSR per step, and the
+    # reference op sequence's (MatMult + 6 BLAS-1 calls: 14 vector streams + CSR)
+    own_bytes_per_col = M * (3 * 8.0 * n + nnz * 12.0 + 4.0 * (n + 1))
+    ref_bytes_per_col = M * (14 * 8.0 * n + nnz * 12.0 + 4.0 * (n + 1))
     iter_s = t / ncol * n
-    return {"value": round(gbs, 3), "unit": "GB/s", "cores": cores, "kind": "port",
+    return {"value": round(gpu_bytes_per_iter / iter_s / 1e9, 3),
+            "unit": "GB/s (the GPU value's algorithmic bytes per SCC iteration / CPU seconds per iteration)",
+            "cores": cores, "kind": "port",
             "sample": f"{ncol} of {n} unit-vector columns, M={M}, FP64 CSR SpMV + BLAS-1 per "
                       f"step (getKPE op sequence), {t:.1f} s",
-            "scc_iter_s_extrapolated": round(iter_s, 1), "nproc": os.cpu_count(), "cpu_model": cpu_model()}
+            "scc_iter_s_extrapolated": round(iter_s, 1),
+            "own_traffic_gbs": round(own_bytes_per_col * ncol / t / 1e9, 1),
+            "reference_op_sequence_gbs": round(ref_bytes_per_col * ncol / t / 1e9, 1),
+            "nproc": os.cpu_count(), "cpu_model": cpu_model()}
 
 
 def cpu_model():
@@ -184,6 +209,75 @@ def time_to_solution(args):
     print(json.dumps(out))
     m.close()
     ctx.close()
+
+
+def tts_leg(ctx, s, args, t32_iter):
+    """Time to solution on the bench structure, both runs to |dq| < 1e-6 e from
+    inputs resident on the device: plain FP64 from the neutral start, and the
+    learned guess (descriptors + network inference inside the clock) with FP32
+    bulk iterations and an FP64 finish.  Also the FP64 Chebyshev kernel's rate.
+    The leg checks the remaining time budget first and skips itself (saying
+    so) rather than run past it; the plain run's iteration cap follows from the
+    budget too."""
+    import numpy as np
+    import largetb_amd as L
+    from largetb_amd import learned
+    EV = 27.211386
+    out = {"workload": f"{s.nat}-atom alpha-quartz {'x'.join(map(str, args.cells))}, M={args.M}, "
+                       f"sigma {args.sigma} A, seed {args.seed}, linear mixing 0.2, |dq| < 1e-6 e"}
+    ctx.set_populations(None)
+    lo, hi = ctx.gershgorin()
+    ctx.rescale((hi - lo) / 2, (hi + lo) / 2)
+    t64, _ = ctx.autotune(L.FP64, 3)  # the FP32 tiling was tuned for the value leg
+    ctx.set_timing(True)
+    w = ctx.scc(args.M, precision=L.FP64, max_iter=1, tol=1e-30, check=False)  # warm-up, eager
+    st = ctx.kpe_stats()
+    ctx.set_timing(False)
+    t64_iter = w["seconds"]
+    k1 = st["bytes"] / st["seconds"] / 1e9
+    out["k1_fp64"] = {"achieved": round(k1, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": round(k1 / HBM_PEAK_GBS, 4), "launches": st["launches"],
+                      "mean_launch_us": round(st["seconds"] / max(st["launches"], 1) * 1e6, 2), "tiling": t64}
+    m, info = learned.load(ctx, args.weights) if args.weights else learned.load(ctx)
+    # iteration counts measured on this workload: 73 plain, 39 learned + mixed
+    est = 1.05 * (76 * t64_iter + 42 * t32_iter + 2.0)
+    left = args.time_budget - (time.perf_counter() - T_START)
+    out["estimate_s"] = round(est, 1)
+    if est > left:
+        out["skipped"] = f"estimated {est:.0f} s exceeds the {left:.0f} s left of the {args.time_budget:.0f} s budget"
+        m.close()
+        return out
+    log("time to solution: plain FP64 from the neutral start")
+    cap = int((left - 1.05 * (42 * t32_iter + 2.0)) / t64_iter)
+    t0 = time.perf_counter()
+    r = ctx.scc(args.M, precision=L.FP64, tol=1e-6, max_iter=min(400, cap), use_graph=1, check=False, verbose=1)
+    plain = dict(seconds=time.perf_counter() - t0, iters=r["iters"], status=r["status"])
+    qp, ep, mp = r["q"], r["eband_fp"], r["mu_ha"]
+    if r["status"] != 0:
+        out["plain_fp64"] = plain
+        out["skipped"] = f"plain FP64 stopped at its budget cap of {cap} iterations"
+        m.close()
+        return out
+    log("time to solution: learned guess + FP32 bulk + FP64 finish")
+    left = args.time_budget - (time.perf_counter() - T_START)
+    t0 = time.perf_counter()
+    guess = m.predict(s)
+    tpred = time.perf_counter() - t0
+    r = ctx.scc(args.M, guess=guess, precision=L.FP32_ACC64, switch_tol=1e-6, tol=1e-6,
+                max_iter=max(1, min(400, int(left / t32_iter) - 2)), use_graph=1, check=False, verbose=1)
+    fast = dict(seconds=time.perf_counter() - t0, iters=r["iters"], fp64_iters=r["fp64_iters"],
+                status=r["status"], predict_s=round(tpred, 3),
+                guess_err_e=float(np.abs(2 * guess - qp).max()))
+    m.close()
+    out["plain_fp64"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in plain.items()}
+    out["learned_mixed"] = {k: (round(v, 3) if isinstance(v, float) and k != "guess_err_e" else v)
+                            for k, v in fast.items()}
+    out["speedup_learned_mixed_vs_plain_fp64"] = round(plain["seconds"] / fast["seconds"], 3)
+    out["max_dq_vs_plain_e"] = float(np.abs(r["q"] - qp).max())
+    out["deband_vs_plain_ev_per_atom"] = abs(r["eband_fp"] - ep) * EV / s.nat
+    out["dmu_vs_plain_ev"] = abs(r["mu_ha"] - mp) * EV
+    out["weights"] = info
+    return out
 
 
 def c1_measure(args):
@@ -495,6 +589,10 @@ def main():
                     help="host vs device neighbour list + H assembly (48x48x24 cells by default)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 1 GiB device-to-device copies (counter calibration)")
+    ap.add_argument("--no-tts", action="store_true",
+                    help="default run: leave out the time-to-solution leg")
+    ap.add_argument("--time-budget", type=float, default=570.0,
+                    help="default run: seconds since start within which the time-to-solution leg must end")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -584,8 +682,11 @@ def main():
                      "tiling": tiling},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.cells, args.sigma, args.seed, args.M)
+        out["cpu_baseline"] = cpu_baseline(args.cells, args.sigma, args.seed, args.M, alg_bytes_iter)
         out["cpu_baseline"]["c0_full_scc"] = c0_scc_both(ctx)
+    if not args.no_tts and prec == L.FP32_ACC64:
+        ctx.load(s)
+        out["time_to_solution"] = tts_leg(ctx, s, args, wall / args.steps)
     ctx.close()
     print(json.dumps(out))
 

@@ -279,6 +279,11 @@ int ltb_kpe_stochastic(ltb_ctx* ctx, int M, int nvec, unsigned long long seed, i
  * ltb_kpe_apply), the same seeded vectors */
 int ltb_density_stochastic(ltb_ctx* ctx, int M, const double* w, int nvec, unsigned long long seed,
                            int precision, double* diagP);
+/* computeAlphas' random-mode trace (:1240-1275) at SP2 depth 0:
+ * trace = sum_r r^T (P r) / nvec over the same seeded vectors (the Hutchinson
+ * estimate of Tr P), P r = sum_m w[m] T_m(H~) r */
+int ltb_trace_stochastic(ltb_ctx* ctx, int M, const double* w, int nvec, unsigned long long seed, int precision,
+                         double* trace);
 /* runKPM's DOS and mu (:1634-1712), host arithmetic on nmom trace moments:
  * Jackson kernel g_n (N = nmom), DOS at x_k = cos(pi (k+1/2)/ntilde), energies
  * a x_k + b; mu where the DOS area from the bottom reaches Ne/(2 nsize) of the

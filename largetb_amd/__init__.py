@@ -124,6 +124,7 @@ SIGNATURES = {
     "ltb_random_vectors": (C.c_int, [C.c_ulonglong, C.c_int, C.c_int, PD]),
     "ltb_kpe_stochastic": (C.c_int, [VP, C.c_int, C.c_int, C.c_ulonglong, C.c_int, PD]),
     "ltb_density_stochastic": (C.c_int, [VP, C.c_int, PD, C.c_int, C.c_ulonglong, C.c_int, PD]),
+    "ltb_trace_stochastic": (C.c_int, [VP, C.c_int, PD, C.c_int, C.c_ulonglong, C.c_int, PD]),
     "ltb_kpm_dos": (C.c_int, [PD, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, PD, PD, PD, PD,
                               PD]),
     "ltb_write_dos": (C.c_int, [C.c_char_p, C.c_int, PD, PD]),
@@ -510,6 +511,13 @@ class Context:
         d = np.zeros(self.norb)
         self._check(lib().ltb_density_stochastic(self._h, M, _d(w), nvec, seed, precision, _d(d)))
         return d
+
+    def trace_stochastic(self, M, w, nvec, seed=1, precision=FP64):
+        """sum_r r^T P r / nvec over the seeded vectors (computeAlphas' trace)"""
+        w = np.ascontiguousarray(w, np.float64)
+        t = C.c_double()
+        self._check(lib().ltb_trace_stochastic(self._h, M, _d(w), nvec, seed, precision, C.byref(t)))
+        return t.value
 
     def keep_orbital_moments(self, on=True):
         self._check(lib().ltb_set_kpe_orbital_moments(self._h, 1 if on else 0))

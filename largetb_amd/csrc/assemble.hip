@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "ctx.hpp"
+#include "quartz_model.hpp"
 
 namespace {
 
@@ -31,39 +32,8 @@ constexpr double kRcut2 = 16.0;  // r^2 < 16 Bohr^2 (:392)
 constexpr double kRmin2 = 0.1;   // r^2 > 0.1 (:421)
 constexpr int kMaxNb = 20;       // maxN_A (:26)
 
-// two-centre fit tables (:75-142), columns (pp-sigma-like, pp-pi-like, sp, ss);
-// [0] Si-Si, [1] O-O, [2] Si-O
-__constant__ double c_tab[3][20][4] = {
-    {{17.865, -5.7398, 17.262, -13.794},   {87.114, 5.8664, 39.685, -7.1328},
-     {-722.33, -52.111, -283.25, 27.816},  {66.059, -3.2651, 34.498, -27.631},
-     {-613.94, -6.6631, -41.298, 12.075},  {-727.27, 1.3757, -112.62, 5.9498},
-     {246.1, -84.273, 241.71, -164.58},    {-1564.3, 4.1541, -179.83, 22.449},
-     {-589.53, -7.9678, 215.3, -36.682},   {-1020.8, 33.73, 7.8809, -87.911},
-     {-812.85, -33.408, 13.727, 158.22},   {1189.7, 49.751, 54.842, -56.155},
-     {-96.931, 8.9531, 102.98, -91.347},   {860.35, 1.2204, -44.669, 12.487},
-     {3551.6, 9.5425, -32.771, 10.438},    {468.97, 1.6056, -128.73, -1.5888},
-     {-317.44, 6.9669, 133.21, 82.532},    {517.74, 47.052, -56.922, 1.1376},
-     {210.8, -18.241, -18.441, 80.163},    {-443.57, 36.135, 52.835, -13.413}},
-    {{5.5526, -0.82118, 3.813, -1.2323},   {-31.886, 10.518, -41.071, 52.454},
-     {233.96, -39.064, 272.9, -407.19},    {-60.015, 61.253, -128.95, -294.22},
-     {213.73, -204.34, 410.72, 1263.6},    {49.054, 32.239, -78.906, -1060},
-     {-73.081, 132.93, -81.269, 1956.2},   {33.537, -13.278, -137.34, -384.47},
-     {78.858, 165.14, -273.61, -873.53},   {-167.41, 68.088, -277.49, 642.65},
-     {75.731, 7.9468, -767.04, -615.24},   {-63.795, -44.453, -288.57, 988.73},
-     {-78.765, 32.997, -76.092, -811.61},  {124.63, -144.7, -189.3, 554.51},
-     {40.615, -149.44, 780.52, 379.23},    {82.116, -29.881, 410.4, -93.407},
-     {-80.656, -54.736, 203.1, -131.08},   {-74.163, 16.814, 710.57, -213.88},
-     {7.3441, 20.02, -100.83, 128.52},     {-93.219, 6.9584, 267.84, -242.74}},
-    {{13.778, -2.7061, 8.6477, -7.6931},   {9.0798, 19.494, -22.43, 52.885},
-     {-15.173, -145.38, 262.06, -468.23},  {9.4007, -23.026, 223.86, -100.16},
-     {40.783, 126.17, -785.64, 228.63},    {23.249, -34.501, 334.88, -253.56},
-     {-17.829, -75.476, 156.18, -638.68},  {18.37, -22.003, 10.705, -312.46},
-     {1.7289, 75.972, 572.55, -1010.6},    {-30.424, 17.518, -55.884, -478.46},
-     {-31.663, -36.928, -349.49, -688.24}, {-82.425, -26.609, -390.25, -296.52},
-     {35.07, 77.948, -143.41, 157.81},     {-27.066, -22.899, -390.79, 580.81},
-     {-20.555, -140.91, -152.48, 401.59},  {49.585, -5.9967, 502.82, 768.76},
-     {-8.7806, 109.97, -113.52, 980.42},   {37.13, -18.019, -148.42, 243.34},
-     {-20.674, 15.652, 904.76, 317.15},    {16.513, 39.501, -630.1, 84.78}}};
+// two-centre fit tables (:75-142): [0] Si-Si, [1] O-O, [2] Si-O
+__constant__ double c_tab[3][20][4] = LTB_SK_TABLES;
 
 struct Geom {
   double cell[9], inv[9];  // rows = periodic vectors; inv = inverse of that matrix
@@ -206,15 +176,15 @@ __device__ void sk_block(int zi, int zj, const double d[3], double h[16]) {
     t = 2;
   } else if (zi == 14) {
     if (r < 0.01) {  // (:258-263)
-      h[0] = -0.39572506;
-      h[5] = h[10] = h[15] = -0.15031380;
+      h[0] = LTB_ONSITE_SI_S;
+      h[5] = h[10] = h[15] = LTB_ONSITE_SI_P;
       return;
     }
     t = 0;
   } else {
     if (r < 0.01) {  // (:299-304)
-      h[0] = -0.87883246;
-      h[5] = h[10] = h[15] = -0.33213167;
+      h[0] = LTB_ONSITE_O_S;
+      h[5] = h[10] = h[15] = LTB_ONSITE_O_P;
       return;
     }
     t = 1;

@@ -107,6 +107,28 @@ int random_diag(State& s, int depth, std::vector<double>& out) {
   return LTB_OK;
 }
 
+// computeAlphas' trace estimate at SP2 depth `depth` (:1240-1275): the mean
+// of r^T P_depth r over opt.nrandoms seeded vectors
+int random_trace(State& s, int depth, double* tr) {
+  const int nR = s.opt.nrandoms, n = s.nSize;
+  const unsigned long long seed = s.opt.seed + 1000003ULL * ++s.draws;
+  if (depth == 0) {
+    std::vector<double> w;
+    CHK(s, kpe_weights(s, w));
+    CHK(s, ltb_trace_stochastic(s.ctx, s.opt.moments, w.data(), nR, seed, s.opt.precision, tr));
+    return LTB_OK;
+  }
+  std::vector<double> R((size_t)nR * n);
+  if (ltb_random_vectors(seed, nR, n, R.data()) != LTB_OK) return LTB_ERR_ARG;
+  double sum = 0;
+  for (int k = 0; k < nR; k++) {
+    Vec r(R.begin() + (size_t)k * n, R.begin() + (size_t)(k + 1) * n), y;
+    CHK(s, getSP2(s, r, y, depth, depth));
+    for (int j = 0; j < n; j++) sum += r[j] * y[j];
+  }
+  *tr = sum / nR;
+  return LTB_OK;
+}
 }  // namespace
 
 PetscErrorCode arrangeCoordinates(State& s) {
@@ -263,9 +285,7 @@ PetscErrorCode computeAlphas(State& s) {
     double tr = 0;
     if (s.opt.nrandoms > 0) {
       // the random-vector trace estimate of the level (:1240-1275)
-      std::vector<double> d;
-      CHK(s, random_diag(s, k, d));
-      for (double x : d) tr += x;
+      CHK(s, random_trace(s, k, &tr));
     } else if (k == 0) {
       // Tr P_KPE from the exact moments (the exact unit-vector trace)
       std::vector<double> d(s.nSize);

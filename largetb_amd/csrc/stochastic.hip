@@ -19,6 +19,7 @@
 // the diagonal estimate is formed on the device as well.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <string>
@@ -93,9 +94,12 @@ __global__ __launch_bounds__(256) void k_gauss_norm2(unsigned long long seed, in
   if (threadIdx.x == 0) r2[k] = sh[0];
 }
 
-// diagP_j = sum_k r_jk (P r)_jk / sum_k r_jk^2 (computeDiagonalElements :1338-1392)
+// ratio: diagP_j = sum_k r_jk (P r)_jk / sum_k r_jk^2 (computeDiagonalElements
+// :1338-1392); otherwise the numerator alone, whose sum over j is
+// sum_k r_k^T P r_k (computeAlphas' trace, :1240-1275)
 __global__ __launch_bounds__(256) void k_stoch_diag(unsigned long long seed, int norb, int nvec, int ld,
-                                                    const double* __restrict__ theta, double* __restrict__ diagP) {
+                                                    const double* __restrict__ theta, int ratio,
+                                                    double* __restrict__ out) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= norb) return;
   double num = 0, den = 0;
@@ -104,7 +108,39 @@ __global__ __launch_bounds__(256) void k_stoch_diag(unsigned long long seed, int
     num += r * theta[(size_t)j * ld + k];
     den += r * r;
   }
-  diagP[j] = num / den;
+  out[j] = ratio ? num / den : num;
+}
+
+// P r for nvec seeded vectors (getKPE on the device-drawn block), then per
+// orbital the ratio estimate of diag(P) or the numerator of the trace
+int stochastic_apply(ltb_ctx* c, int M, const double* w, int nvec, unsigned long long seed, int precision, int ratio,
+                     std::vector<double>& out) {
+  const int norb = c->norb, ld = (nvec + 63) / 64 * 64;
+  const bool f64 = precision == LTB_FP64;
+  const size_t n = (size_t)norb * ld;
+  DevBuf a, b, th, d;
+  LTB_HIP(c, hipMalloc(&a.p, n * (f64 ? 8 : 4)));
+  LTB_HIP(c, hipMalloc(&b.p, n * (f64 ? 8 : 4)));
+  LTB_HIP(c, hipMalloc(&th.p, n * sizeof(double)));
+  LTB_HIP(c, hipMalloc(&d.p, sizeof(double) * norb));
+  const dim3 grid(ld / 64, (norb + 3) / 4);
+  if (f64)
+    hipLaunchKernelGGL(k_gauss_block<double>, grid, dim3(256), 0, c->stream, seed, norb, nvec, ld, (double*)a.p);
+  else
+    hipLaunchKernelGGL(k_gauss_block<float>, grid, dim3(256), 0, c->stream, seed, norb, nvec, ld, (float*)a.p);
+  LTB_HIP(c, hipGetLastError());
+  int st = kpe_apply_device(c, M, w, ld, a.p, b.p, (double*)th.p, precision);
+  if (st) {
+    (void)hipStreamSynchronize(c->stream);  // the buffers are freed on return
+    return st;
+  }
+  hipLaunchKernelGGL(k_stoch_diag, dim3((norb + 255) / 256), dim3(256), 0, c->stream, seed, norb, nvec, ld,
+                     (const double*)th.p, ratio, (double*)d.p);
+  LTB_HIP(c, hipGetLastError());
+  out.resize(norb);
+  LTB_HIP(c, hipMemcpyAsync(out.data(), d.p, sizeof(double) * norb, hipMemcpyDeviceToHost, c->stream));
+  LTB_HIP(c, hipStreamSynchronize(c->stream));
+  return LTB_OK;
 }
 
 }  // namespace
@@ -205,30 +241,24 @@ int ltb_density_stochastic(ltb_ctx* c, int M, const double* w, int nvec, unsigne
   if (!c || M < 1 || !w || !diagP || nvec < 1 || (precision != LTB_FP64 && precision != LTB_FP32_ACC64))
     return LTB_ERR_ARG;
   if (c->nat <= 0) return set_err(c, LTB_ERR_STATE, "no system loaded");
-  const int norb = c->norb, ld = (nvec + 63) / 64 * 64;
-  const bool f64 = precision == LTB_FP64;
-  const size_t n = (size_t)norb * ld;
-  DevBuf a, b, th, d;
-  LTB_HIP(c, hipMalloc(&a.p, n * (f64 ? 8 : 4)));
-  LTB_HIP(c, hipMalloc(&b.p, n * (f64 ? 8 : 4)));
-  LTB_HIP(c, hipMalloc(&th.p, n * sizeof(double)));
-  LTB_HIP(c, hipMalloc(&d.p, sizeof(double) * norb));
-  const dim3 grid(ld / 64, (norb + 3) / 4);
-  if (f64)
-    hipLaunchKernelGGL(k_gauss_block<double>, grid, dim3(256), 0, c->stream, seed, norb, nvec, ld, (double*)a.p);
-  else
-    hipLaunchKernelGGL(k_gauss_block<float>, grid, dim3(256), 0, c->stream, seed, norb, nvec, ld, (float*)a.p);
-  LTB_HIP(c, hipGetLastError());
-  int st = kpe_apply_device(c, M, w, ld, a.p, b.p, (double*)th.p, precision);
-  if (st) {
-    (void)hipStreamSynchronize(c->stream);  // the buffers are freed on return
-    return st;
-  }
-  hipLaunchKernelGGL(k_stoch_diag, dim3((norb + 255) / 256), dim3(256), 0, c->stream, seed, norb, nvec, ld,
-                     (const double*)th.p, (double*)d.p);
-  LTB_HIP(c, hipGetLastError());
-  LTB_HIP(c, hipMemcpyAsync(diagP, d.p, sizeof(double) * norb, hipMemcpyDeviceToHost, c->stream));
-  LTB_HIP(c, hipStreamSynchronize(c->stream));
+  std::vector<double> d;
+  const int st = stochastic_apply(c, M, w, nvec, seed, precision, 1, d);
+  if (st) return st;
+  std::copy(d.begin(), d.end(), diagP);
+  return LTB_OK;
+}
+
+int ltb_trace_stochastic(ltb_ctx* c, int M, const double* w, int nvec, unsigned long long seed, int precision,
+                         double* trace) {
+  if (!c || M < 1 || !w || !trace || nvec < 1 || (precision != LTB_FP64 && precision != LTB_FP32_ACC64))
+    return LTB_ERR_ARG;
+  if (c->nat <= 0) return set_err(c, LTB_ERR_STATE, "no system loaded");
+  std::vector<double> num;
+  const int st = stochastic_apply(c, M, w, nvec, seed, precision, 0, num);
+  if (st) return st;
+  double s = 0;
+  for (double x : num) s += x;
+  *trace = s / nvec;
   return LTB_OK;
 }
 

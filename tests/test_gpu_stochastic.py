@@ -90,6 +90,27 @@ def test_stochastic_diagonal(ctx, c0):
     assert abs(got.sum() - d["nelec"] / 2) < 0.1 * s.norb
 
 
+def test_stochastic_trace_is_the_mean_of_r_P_r(ctx, c0):
+    """computeAlphas' random-mode trace (withSP2_deltaH.cpp:1240-1275):
+    sum_r r^T P r / nRandoms over the seeded vectors, against the dense
+    formula; and with many vectors it estimates Tr P within 5 sigma
+    (sigma^2 = 2 ||P||_F^2 / nvec for Gaussian vectors)."""
+    s, o, a, b, Ht = c0
+    M = 200
+    ctx.kpe_moments(M, L.FP64)
+    ctx.density(M)
+    w = ctx.kpe_weights(M)
+    e, V = np.linalg.eigh(Ht)
+    P = (V * np.polynomial.chebyshev.chebval(e, w)) @ V.T
+    for nvec, seed in ((32, 3), (500, 11)):
+        got = ctx.trace_stochastic(M, w, nvec, seed=seed)
+        R = L.random_vectors(seed, nvec, s.norb)
+        ref = np.einsum("ij,jk,ik->", R, P, R) / nvec
+        assert abs(got - ref) < 1e-9 * abs(ref)
+    sigma = np.sqrt(2 * (P * P).sum() / 500)
+    assert abs(got - np.trace(P)) < 5 * sigma
+
+
 @pytest.mark.parametrize("prec,tol", [(L.FP64, 1e-11), (L.FP32_ACC64, 1e-5)])
 def test_device_draws_match_host_stream_at_size(ctx, prec, tol):
     """18,432 orbitals: the seeded entry points (vectors drawn on the device,
