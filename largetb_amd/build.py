@@ -63,6 +63,20 @@ def build(jobs=4):
     return LIB
 
 
+def build_host_check(sanitize=True):
+    """The host-only self-check (csrc/host_check.cpp) over system.cpp and
+    matlab_io.cpp, with AddressSanitizer and UndefinedBehaviorSanitizer (g++,
+    no device code); returns the executable's path."""
+    os.makedirs(OBJ, exist_ok=True)
+    exe = os.path.join(OBJ, "host_check_asan" if sanitize else "host_check")
+    flags = ["-std=c++17", "-g", "-O1", "-Wall"]
+    if sanitize:
+        flags += ["-fsanitize=address,undefined", "-fno-sanitize-recover=undefined", "-fno-omit-frame-pointer"]
+    srcs = [os.path.join(CSRC, f) for f in ("host_check.cpp", "system.cpp", "matlab_io.cpp")]
+    _run(["g++"] + flags + ["-o", exe] + srcs)
+    return exe
+
+
 def _wait(item):
     cmd, p = item
     out, err = p.communicate()
