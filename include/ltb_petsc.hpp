@@ -36,6 +36,9 @@ struct Settings {
   int quad_nodes = 8192;
   int maxsp2 = 1;                       // computeDensityMatrix (:1402)
   double force_alpha = 0.0;             // != 0: every SP2 level uses this alpha instead of computeAlphas' rule
+  std::vector<double> force_alphas;     // non-empty: level k uses force_alphas[min(k, size - 1)] (overrides)
+  double kpe_scale = 1.0;               // getKPE's output scale: 1 as here (:1068-1185); 2 as withSP2.cpp's
+                                        // getKPE, which returns 2 P x, the spin-summed density (SP2:939)
   int precision = LTB_FP64;
   double mix = 0.2;                     // linear charge mixing (replaces mixRatio = 0.0, :56)
   double tol = 1e-6;                    // |dq|inf in e (reference: 0.1, :1839)

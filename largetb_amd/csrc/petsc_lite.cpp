@@ -91,6 +91,7 @@ int random_diag(State& s, int depth, std::vector<double>& out) {
     std::vector<double> w;
     CHK(s, kpe_weights(s, w));
     CHK(s, ltb_density_stochastic(s.ctx, s.opt.moments, w.data(), nR, seed, s.opt.precision, out.data()));
+    for (double& x : out) x *= s.opt.kpe_scale;
     return LTB_OK;
   }
   std::vector<double> R((size_t)nR * n), num(n, 0.0), den(n, 0.0);
@@ -116,6 +117,7 @@ int random_trace(State& s, int depth, double* tr) {
     std::vector<double> w;
     CHK(s, kpe_weights(s, w));
     CHK(s, ltb_trace_stochastic(s.ctx, s.opt.moments, w.data(), nR, seed, s.opt.precision, tr));
+    *tr *= s.opt.kpe_scale;
     return LTB_OK;
   }
   std::vector<double> R((size_t)nR * n);
@@ -258,6 +260,8 @@ PetscErrorCode getKPE(State& s, const Vec& in, Vec& out) {
   CHK(s, kpe_weights(s, w));
   out.assign(s.nSize, 0.0);
   CHK(s, ltb_kpe_apply(s.ctx, s.opt.moments, w.data(), 1, in.data(), out.data(), LTB_FP64));
+  if (s.opt.kpe_scale != 1.0)
+    for (double& x : out) x *= s.opt.kpe_scale;  // withSP2.cpp's VecScale(outVector, 2.0) (SP2:939)
   return LTB_OK;
 }
 
@@ -277,6 +281,11 @@ PetscErrorCode getSP2(State& s, const Vec& in, Vec& out, int localsp2max, int cu
 }
 
 PetscErrorCode computeAlphas(State& s) {
+  if (!s.opt.force_alphas.empty()) {
+    const size_t n = s.opt.force_alphas.size();
+    for (size_t k = 0; k < s.alphavals.size(); k++) s.alphavals[k] = s.opt.force_alphas[std::min(k, n - 1)];
+    return LTB_OK;
+  }
   if (s.opt.force_alpha != 0.0) {
     for (double& a : s.alphavals) a = s.opt.force_alpha;
     return LTB_OK;
@@ -290,7 +299,7 @@ PetscErrorCode computeAlphas(State& s) {
       // Tr P_KPE from the exact moments (the exact unit-vector trace)
       std::vector<double> d(s.nSize);
       CHK(s, ltb_density_diag(s.ctx, d.data()));
-      for (double x : d) tr += x;
+      for (double x : d) tr += s.opt.kpe_scale * x;
     } else {
       std::vector<double> d(s.nSize);
       CHK(s, diag_block(s, k, 0, s.nSize, d));
@@ -308,9 +317,10 @@ PetscErrorCode computeDiagonalElements(State& s) {
   s.diagP.assign(s.nSize, 0.0);
   if (s.opt.nrandoms > 0)
     CHK(s, random_diag(s, depth, s.diagP));
-  else if (depth == 0)
+  else if (depth == 0) {
     CHK(s, ltb_density_diag(s.ctx, s.diagP.data()));
-  else
+    for (double& x : s.diagP) x *= s.opt.kpe_scale;
+  } else
     CHK(s, diag_block(s, depth, 0, s.nSize, s.diagP));
   // chargeOnAtom = orbital2Atom . diagP (:1385)
   s.chargeOnAtom.assign(s.Natoms, 0.0);

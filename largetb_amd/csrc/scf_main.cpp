@@ -4,7 +4,7 @@
 // mu and the random-vector diagonal are used, as in the reference.
 //   ltb_scf L M N [nRandoms] [--sigma A] [--seed S] [--moments M] [--kT eV] [--reference]
 //                 [--maxsp2 D] [--tol Q] [--max-scf K] [--fp32] [--sp2-alpha a] [--dump DIR]
-//                 [--mix m] [--reference-literal]
+//                 [--mix m] [--reference-literal] [--sp2-alphas a1,a2,..] [--kpe-times-two]
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -22,7 +22,8 @@ static double now() {
 int main(int argc, char** argv) {
   if (argc < 4) {
     std::fprintf(stderr, "usage: %s L M N [nRandoms] [--sigma A] [--seed S] [--moments M] [--kT eV] "
-                 "[--reference] [--maxsp2 D] [--tol Q] [--max-scf K] [--fp32] [--sp2-alpha a] [--dump DIR] [--mix m] [--reference-literal]\n", argv[0]);
+                 "[--reference] [--maxsp2 D] [--tol Q] [--max-scf K] [--fp32] [--sp2-alpha a] [--sp2-alphas a1,a2,..] [--kpe-times-two] [--dump DIR] [--mix m] "
+                 "[--reference-literal]\n", argv[0]);
     return 2;
   }
   State s;
@@ -47,6 +48,14 @@ int main(int argc, char** argv) {
     else if (a == "--max-scf") s.opt.max_scf = std::atoi(val());
     else if (a == "--fp32") s.opt.precision = LTB_FP32_ACC64;
     else if (a == "--sp2-alpha") s.opt.force_alpha = std::atof(val());
+    else if (a == "--sp2-alphas") {  // comma-separated, one per SP2 level
+      std::string list = val();
+      for (size_t p = 0; p <= list.size();) {
+        const size_t q = std::min(list.find(',', p), list.size());
+        s.opt.force_alphas.push_back(std::atof(list.substr(p, q - p).c_str()));
+        p = q + 1;
+      }
+    } else if (a == "--kpe-times-two") s.opt.kpe_scale = 2.0;
     else if (a == "--dump") s.opt.dump_dir = val();
     else if (a == "--mix") s.opt.mix = std::atof(val());
     else if (a == "--reference-literal") s.opt.reference_literal = true;

@@ -44,13 +44,48 @@ def test_scf_driver_matches_oracle(tmp_path):
     assert out["total_charge"] == pytest.approx(384.0, abs=1e-8)
 
 
-def p0_eigen():
+def p0_eigen(M=200):
     o = O.System.quartz(2, 2, 2, sigma_ang=0.05, seed=1)
-    d = o.density(200)
+    d = o.density(M)
     e, V = np.linalg.eigh(o.dense_h())
-    c = O.fd_coeffs(200, d["mu"], 0.05 / O.HARTREE_EV / d["a"], 8192)
-    f = np.polynomial.chebyshev.chebval((e - d["b"]) / d["a"], O.kpe_weights(c, O.gmM(200)))
+    c = O.fd_coeffs(M, d["mu"], 0.05 / O.HARTREE_EV / d["a"], 8192)
+    f = np.polynomial.chebyshev.chebval((e - d["b"]) / d["a"], O.kpe_weights(c, O.gmM(M)))
     return o, V, f
+
+
+def sp2_levels(f, alphas):
+    """getSP2's recursion on eigenvalues: level k maps x -> (1 + a_k) x - a_k x^2"""
+    for a in alphas:
+        f = (1 + a) * f - a * f * f
+    return f
+
+
+@pytest.mark.gpu
+def test_sp2_depth_four_through_getSP2(tmp_path):
+    """maxsp2 = 5, withSP2.cpp's depth (SP2:1110), with the level pattern
+    +1, -1, +1, -1: sixteen getKPE calls per unit vector through the recursive
+    getSP2 (exact mode), against the SP2 polynomial of the KPE density's
+    eigenvalues."""
+    r, out, q = run(["2", "2", "2", "--sigma", "0.05", "--seed", "1", "--moments", "100",
+                     "--maxsp2", "5", "--max-scf", "1", "--sp2-alphas", "1,-1,1,-1"], tmp_path, timeout=600)
+    assert r.returncode in (0, 5), r.stderr
+    o, V, f = p0_eigen(100)
+    pop = ((V ** 2) @ sp2_levels(f, [1, -1, 1, -1])).reshape(-1, 4).sum(1)
+    assert np.abs(q - 2 * pop).max() < 1e-8
+
+
+@pytest.mark.gpu
+def test_sp2_with_the_doubled_kpe_output(tmp_path):
+    """withSP2.cpp's getKPE returns 2 P x (VecScale(outVector, 2.0), SP2:939):
+    --kpe-times-two at depth 2 (levels -1, +1) gives the SP2 polynomial of
+    2 P0."""
+    r, out, q = run(["2", "2", "2", "--sigma", "0.05", "--seed", "1", "--moments", "100",
+                     "--maxsp2", "3", "--max-scf", "1", "--sp2-alphas", "-1,1", "--kpe-times-two"], tmp_path,
+                    timeout=600)
+    assert r.returncode in (0, 5), r.stderr
+    o, V, f = p0_eigen(100)
+    pop = ((V ** 2) @ sp2_levels(2 * f, [-1, 1])).reshape(-1, 4).sum(1)
+    assert np.abs(q - 2 * pop).max() < 1e-8 * max(1.0, np.abs(pop).max())
 
 
 @pytest.mark.gpu
