@@ -11,6 +11,8 @@
   schedule, and its partial-row clamp).
 * The 12x12x12 density and two SCC iterations in both precisions.
 * The 256-cell batch (2x2x2 .. 5x5x4, seed 1) at full size.
+* The largest single-GPU cell (16x16x12, 27,648 atoms) at M = 1024 in both
+  precisions.
 
 Tolerances: FP64 moments agree with the FP64 oracle to 1e-11 (summation
 order only); FP32 recursion (FP32 vectors and H~, FP64 accumulation) to
@@ -219,3 +221,24 @@ def test_batch_of_256_cells_at_full_size(ctx):
     ref = O.System.quartz(l, m, n, sigma_ang=sg, seed=sd).scc(256, tol=1e-6, maxit=400)
     assert ref["status"] == 0
     assert np.abs(qs[k] - ref["q"]).max() < 1e-6
+
+
+def test_largest_cell_density_at_M1024_fp32_against_fp64(ctx):
+    """The largest single-GPU cell (16x16x12, 27,648 atoms, 110,592 orbitals,
+    sigma 0.08 A) at its configured M = 1024: the neutral-start density in
+    both precisions with their autotuned tilings; FP32 charges within 2e-6 e
+    and band energy within 1e-6 eV/atom of FP64, electrons conserved."""
+    s = L.System.quartz(16, 16, 12, sigma_ang=0.08, seed=1)
+    prepare(ctx, s)
+    d = {}
+    for prec in (L.FP64, L.FP32_ACC64):
+        ctx.autotune(prec, 2)
+        am, tot = ctx.kpe_moments(1024, prec)
+        assert tot[0] == s.norb and np.all(am[0] == 4.0)
+        d[prec] = ctx.density(1024)
+    dq = 2 * np.abs(d[L.FP32_ACC64]["pop"] - d[L.FP64]["pop"]).max()
+    deb = abs(d[L.FP32_ACC64]["eband"] - d[L.FP64]["eband"]) * EV / s.nat
+    print(f"27,648 atoms, M=1024: max |dq| {dq:.3e} e, |dE_band| {deb:.3e} eV/atom")
+    assert dq < 2e-6 and deb < 1e-6
+    for r in d.values():
+        assert r["nelec"] == pytest.approx(s.nelectrons, rel=1e-12)
