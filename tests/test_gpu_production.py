@@ -136,6 +136,24 @@ def test_production_fp32_against_fp64(request, cell):
         assert d["nelec"] == pytest.approx(p.s.nelectrons, rel=1e-12)
 
 
+def test_c1_scc_mixed_precision_against_plain_fp64(ctx, c1):
+    """The whole 4,608-atom SCC to |dq| < 1e-6 e (hipGraph iterations): FP32
+    bulk with the FP64 finish against the plain FP64 solve -- every charge
+    within 1e-6 e, the fixed-point band energy within 1e-6 eV/atom, and both
+    conserve the electrons."""
+    ctx.load(c1.s)
+    ref = ctx.scc(M, precision=L.FP64, tol=1e-6, max_iter=200, use_graph=1)
+    mix = ctx.scc(M, precision=L.FP32_ACC64, switch_tol=1e-6, tol=1e-6, max_iter=200, use_graph=1)
+    dq = np.abs(mix["q"] - ref["q"]).max()
+    deb = abs(mix["eband_fp"] - ref["eband_fp"]) * EV / c1.s.nat
+    print(f"c1 SCC: plain {ref['iters']} iterations, mixed {mix['iters']} ({mix['fp64_iters']} FP64): "
+          f"max |dq| {dq:.3e} e, |dE_band| {deb:.3e} eV/atom")
+    assert ref["status"] == 0 and mix["status"] == 0 and 1 <= mix["fp64_iters"] <= 5
+    assert dq < 1e-6 and deb < 1e-6
+    for r in (ref, mix):
+        assert r["q"].sum() == pytest.approx(c1.s.nelectrons, abs=1e-6)
+
+
 def test_c3_scc_iterations_fp32_against_fp64(ctx, c3):
     """Two SCC iterations from the neutral start in each precision: the
     mixed charges and the charges of the second density agree to 2e-6 e."""
