@@ -52,7 +52,7 @@ struct alignas(64) BatchTask {
   int s;            // structure, -1 for a padding slot
   int a0;           // its first atom (global block row)
   int nat;          // its atoms
-  int ld;           // its row stride (columns of all its groups)
+  int ld;           // its row stride in elements (a power of two >= the columns of all its groups)
   int cstart;       // first column of the group
   int abase;        // first atom of the task, local to the structure
   int width;        // 256, 128 or 64 columns
@@ -468,7 +468,8 @@ void kernel_gmM(int M, std::vector<double>& g) {
 struct ltb_batch {
   ltb_ctx* ctx = nullptr;
   int S = 0, nat = 0, nnzb = 0, ntasks = 0, ncols = 0;
-  std::vector<int> aoff, lds;
+  std::vector<int> aoff, lds;  // per structure: atom offsets, columns of all its groups
+  std::vector<int> strides;    // per structure: row stride of its vectors (lds rounded up to a power of two)
   std::vector<size_t> xoff;
   std::vector<void*> allocs;
   int *d_rowptr, *d_col, *d_blkrow, *d_atomst, *d_grow, *d_gcol, *d_aoff, *d_active;
@@ -582,7 +583,7 @@ int build_tasks(ltb_batch* b) {
         t.s = s;
         t.a0 = b->aoff[s];
         t.nat = nat;
-        t.ld = ld;
+        t.ld = b->strides[s];
         t.cstart = c0;
         t.abase = g * 4 * apw;
         t.width = width;
@@ -787,8 +788,11 @@ int ltb_batch_create(ltb_ctx* c, int nstruct, const ltb_system* const* systems, 
   b->xoff.push_back(0);
   for (int s = 0; s < nstruct; s++) {
     const int nat = b->aoff[s + 1] - b->aoff[s], norb = 4 * nat, ld = (norb & ~255) + tail_width(norb);
+    int stride = 256;
+    while (stride < ld) stride *= 2;
     b->lds.push_back(ld);
-    b->xoff.push_back(b->xoff.back() + (size_t)norb * ld);
+    b->strides.push_back(stride);
+    b->xoff.push_back(b->xoff.back() + (size_t)norb * stride);
     for (int cc = 0; cc < ld; cc++) {
       gc_s.push_back(s);
       gc_c.push_back(cc);
