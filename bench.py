@@ -327,6 +327,7 @@ def c1_measure(args):
         peak = 78.6 if name == "fp64" else 157.3
         tf = r["gemm_flops"] / r["gemm_seconds"] / 1e12
         out[f"sp2_{name}"] = {"iters": r["iters"], "fp64_iters": r["fp64_iters"], "seconds": round(r["wall"], 3),
+                              "gemm_ms_per_product": round(1e3 * r["gemm_seconds"] / r["iters"], 2),
                               "gemm_tflops_exec": round(tf, 2), "trace_minus_nocc": r["trace"] - s.nelectrons / 2,
                               "idem": r["idem"], "mfma_peak_tflops": peak}
     out["sp2_fp32_vs_fp64"] = {"max_diag_diff": float(np.abs(sp["fp64"]["diag"] - sp["fp32_fp64tail"]["diag"]).max()),
@@ -373,6 +374,8 @@ def sp2_measure(args):
     out = {"workload": f"dense SP2, {s.nat}-atom alpha-quartz {'x'.join(map(str, cells))}, {n} orbitals",
            "iterations": r["iters"], "fp64_iterations": r["fp64_iters"], "seconds": round(wall, 3),
            "gemm_seconds": round(r["gemm_seconds"], 4),
+           "gemm_ms_per_product": round(1e3 * r["gemm_seconds"] / r["iters"], 2),
+           # 2 n^3 per X^2 over the time: the rate a full (unsymmetric) product would need
            "gemm_tflops_alg": round(2.0 * n ** 3 * r["iters"] / r["gemm_seconds"] / 1e12, 2),
            "gemm_tflops_exec": round(r["gemm_flops"] / r["gemm_seconds"] / 1e12, 2),
            "trace_minus_nocc": r["trace"] - s.nelectrons / 2, "idem": r["idem"],

@@ -325,7 +325,9 @@ typedef struct {
   double trace;   /* tr X at the end (per spin)                  */
   double idem;    /* |tr X - tr X^2| of the final iterate          */
   double gemm_seconds; /* time inside the X^2 products          */
-  double gemm_flops;   /* 2 n^3 per product, summed             */
+  double gemm_flops;   /* MFMA flops executed, summed: 2 x 128^2 x n per formed
+                          128 x 128 tile, n = norb rounded up to 128; the tiles on
+                          and above the diagonal are formed (X^2 is symmetric) */
 } ltb_sp2_result;
 int ltb_sp2_dense(ltb_ctx* ctx, double Ne, int max_iter, int fp64_tail, double* diagP, double* pop,
                   ltb_sp2_result* res);

@@ -11,11 +11,13 @@
 //
 // GEMM: X is symmetric, so X^2 = X^T X and both operand tiles are row slabs
 // X[k0:k0+16, i0:i0+128] and X[k0:k0+16, j0:j0+128] -- contiguous 512 B (FP32)
-// rows, and every C[i,j], C[j,i] pair is formed from the same products in the
-// same order (the iterate stays exactly symmetric).  Workgroup = 256 threads
-// (2x2 waves), 128x128 output tile, K-slab 16 staged through LDS with a
-// register prefetch of the next slab.  The epilogue applies the branch and
-// accumulates tr(X^2) from the diagonal tiles (one FP64 atomic per wave).
+// rows.  Only the tiles on and above the diagonal are formed (nt (nt+1) / 2 of
+// the nt^2, nt = N / 128); each off-diagonal tile is also written to its mirror
+// position, so the iterate stays exactly symmetric and the MFMA work is
+// (nt + 1) / (2 nt) of the full product.  Workgroup = 256 threads (2x2 waves),
+// 128x128 output tile, K-slab 16 staged through LDS with a register prefetch of
+// the next slab.  The epilogue applies the branch and accumulates tr(X^2) from
+// the diagonal tiles (one FP64 atomic per wave).
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -50,6 +52,7 @@ struct Mfma<float> {
   // accumulator e of lane: row 8*(e/4) + 4*(lane/32) + e%4, column lane%32
   __device__ static int acc_row(int lane, int e) { return 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3); }
   __device__ static int acc_col(int lane) { return lane & 31; }
+  static constexpr int RUN = 4;  // accumulators e..e+3 (e % 4 == 0) hold consecutive rows
 };
 
 template <>
@@ -66,12 +69,26 @@ struct Mfma<double> {
   // accumulator e of lane: row 4*e + lane/16, column lane%16 (checked by a lane probe)
   __device__ static int acc_row(int lane, int e) { return 4 * e + (lane >> 4); }
   __device__ static int acc_col(int lane) { return lane & 15; }
+  static constexpr int RUN = 1;
 };
 
+// upper-triangle tile t (row-major over ti <= tj) -> (ti, tj); nt tiles per edge
+__device__ inline void upper_tile(int t, int nt, int& ti, int& tj) {
+  // row ti starts at s(ti) = ti nt - ti (ti - 1) / 2; invert, then correct the rounding
+  const double b = 2.0 * nt + 1.0;
+  int r = (int)((b - sqrt(b * b - 8.0 * t)) * 0.5);
+  auto start = [nt](int q) { return q * nt - q * (q - 1) / 2; };
+  while (r > 0 && start(r) > t) r--;
+  while (r + 1 < nt && start(r + 1) <= t) r++;
+  ti = r;
+  tj = r + (t - start(r));
+}
+
 // Y = X^2 (square = 1) or 2X - X^2 (square = 0); trC += tr(X^2).
-// N is a multiple of 128 (zero padding); ld = N.
+// N is a multiple of 128 (zero padding); ld = N; one workgroup per tile on or
+// above the diagonal (grid = nt (nt + 1) / 2).
 template <typename T>
-__global__ __launch_bounds__(256, 2) void k_sp2_gemm(const T* __restrict__ X, T* __restrict__ Y, int N,
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 3 : 2) void k_sp2_gemm(const T* __restrict__ X, T* __restrict__ Y, int N,
                                                   int square, double* __restrict__ trC) {
   using MF = Mfma<T>;
   constexpr int BK = 16;
@@ -81,7 +98,9 @@ __global__ __launch_bounds__(256, 2) void k_sp2_gemm(const T* __restrict__ X, T*
   constexpr int NV = EPT / VEC;                     // 16-byte loads per thread per slab
   __shared__ T sA[2][BK][BM + PAD];
   __shared__ T sB[2][BK][BM + PAD];
-  const int i0 = blockIdx.y * BM, j0 = blockIdx.x * BM;
+  int ti, tj;
+  upper_tile(blockIdx.x, N / BM, ti, tj);
+  const int i0 = ti * BM, j0 = tj * BM;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
   // slab load map: thread -> (row r, columns c..c+EPT)
@@ -140,20 +159,37 @@ __global__ __launch_bounds__(256, 2) void k_sp2_gemm(const T* __restrict__ X, T*
   }
 
   double dsum = 0;
+  const bool mirror = ti != tj;
+  typedef T run_t __attribute__((ext_vector_type(MF::RUN)));
 #pragma unroll
   for (int a = 0; a < TPW; a++)
 #pragma unroll
-    for (int b = 0; b < TPW; b++)
+    for (int b = 0; b < TPW; b++) {
+      const int j = j0 + wc * 64 + b * MF::TM + MF::acc_col(lane);
 #pragma unroll
       for (int e = 0; e < MF::NACC; e++) {
         const int i = i0 + wr * 64 + a * MF::TM + MF::acc_row(lane, e);
-        const int j = j0 + wc * 64 + b * MF::TM + MF::acc_col(lane);
         const T c = acc[a][b][e];
         const size_t o = (size_t)i * N + j;
         Y[o] = square ? c : T(2) * X[o] - c;
         if (i == j) dsum += (double)c;
       }
-  if (blockIdx.x == blockIdx.y && wr == wc) {
+      if (mirror) {
+        // C[j, i] = C[i, j]: rows acc_row(e .. e+RUN-1) are consecutive, so each
+        // lane writes RUN consecutive elements of row j of Y
+#pragma unroll
+        for (int e = 0; e < MF::NACC; e += MF::RUN) {
+          const int i = i0 + wr * 64 + a * MF::TM + MF::acc_row(lane, e);
+          const size_t o = (size_t)j * N + i;
+          run_t c;
+#pragma unroll
+          for (int q = 0; q < MF::RUN; q++) c[q] = acc[a][b][e + q];
+          if (!square) c = T(2) * *reinterpret_cast<const run_t*>(X + o) - c;
+          *reinterpret_cast<run_t*>(Y + o) = c;
+        }
+      }
+    }
+  if (ti == tj && wr == wc) {
     for (int off = 32; off > 0; off >>= 1) dsum += __shfl_xor(dsum, off);
     if (lane == 0) atomicAdd(trC, dsum);
   }
@@ -245,7 +281,8 @@ int sp2_dense_impl(ltb_ctx* c, double Ne, int max_iter, int fp64_tail, double* d
     for (int i = 0; i < norb; i++) trH += d0[i] + v[i / 4];
   }
   double trX = (emax * norb - trH) / (emax - emin);
-  dim3 grid(N / BM, N / BM), block(256);
+  const int nt = N / BM;
+  dim3 grid(nt * (nt + 1) / 2), block(256);
   // branch rule: the reference's (square iff 2 tr X > Ne, computeAlphas :1277-1281)
   // in the bulk; in the FP64 tail after an FP32 bulk, Niklasson's rule (square
   // iff |tr X^2 - Nocc| < |2 tr X - tr X^2 - Nocc|), which also contracts
@@ -280,7 +317,7 @@ int sp2_dense_impl(ltb_ctx* c, double Ne, int max_iter, int fp64_tail, double* d
       float ms = 0;
       LTB_HIP(c, hipEventElapsedTime(&ms, e0, e1));
       gemm_s += ms * 1e-3;
-      flops += 2.0 * (double)N * N * N;
+      flops += 2.0 * BM * BM * (double)N * (double)grid.x;  // MFMA work of the formed tiles
       const double idem = std::fabs(trX - trC);
       trX = square ? trC : 2.0 * trX - trC;
       std::swap(A, B);
