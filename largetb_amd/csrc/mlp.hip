@@ -17,8 +17,20 @@
 // Parameter vector (floats): per species [W1 64x80 | b1 | W2 64x64 | b2 |
 // W3 64x64 | b3 | w4 64 | b4] (13569 each, trainable), then mean[2][80],
 // sd[2][80], shift[2] (fixed after fit_normalization).
+//
+// Evaluation on MFMA: the atoms of one species form tiles of 64; a workgroup
+// (4 waves) stages its species' weights and one tile's normalised features in
+// LDS and forms every 64-wide layer as a 64 x 64 product on
+// v_mfma_f32_32x32x2f32, one 32 x 32 quadrant per wave.  The backward pass
+// recomputes the forward pass of its tile (no per-atom activations are kept
+// in HBM), overwrites z3, z2, z1 in LDS with the layer gradients and adds
+// the tile's weight gradients to MFMA accumulators that stay in registers
+// over all the tiles of a persistent workgroup; every workgroup then writes
+// one partial gradient and a fixed-order sum forms the gradient (the result
+// does not depend on timing).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -35,7 +47,9 @@ constexpr int NTRAIN = 2 * PS;
 constexpr int OMEAN = NTRAIN, OSD = OMEAN + 2 * NF, OSHIFT = OSD + 2 * NF, NPARAM = OSHIFT + 2;
 constexpr double kPi = 3.141592653589793238463;
 constexpr double kRcAng = 6.0, kR0Ang = 0.5;
-constexpr int NPART = 64;  // atom partitions of the weight-gradient reduction
+constexpr int NPB = 256;   // persistent workgroups of the backward pass (partial gradients)
+constexpr int TA = 64;     // atoms per tile
+constexpr int LX = NF + 1, LH = NH + 1;  // odd LDS row strides: row and column reads are conflict-free
 
 __device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
 __device__ __forceinline__ float dsilu(float z) {
@@ -102,45 +116,262 @@ __global__ __launch_bounds__(128) void k_descriptors(int nat, const int* __restr
   for (int f = 0; f < NF; f++) feat[(size_t)i * NF + f] = G[f];
 }
 
-__global__ __launch_bounds__(128) void k_mlp_forward(int n, const float* __restrict__ feat, const int* __restrict__ Z,
-                              const float* __restrict__ P, float* __restrict__ z1,
-                              float* __restrict__ z2, float* __restrict__ z3,
-                              double* __restrict__ qraw) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= n) return;
-  const int c = species(Z[a]);
+typedef float acc16 __attribute__((ext_vector_type(16)));
+__device__ __forceinline__ acc16 mma32(float a, float b, acc16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+}
+// v_mfma_f32_32x32x2f32 lane map: A[m = lane % 32][k = lane / 32],
+// B[k = lane / 32][n = lane % 32]; accumulator e of a lane holds row
+// 8 (e / 4) + 4 (lane / 32) + e % 4 and column lane % 32 of the 32 x 32 tile.
+__device__ __forceinline__ int acc_row(int lane, int e) { return 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3); }
+
+struct MlpSmem {
+  float W1[NH * LX];  // W1[o][f] at o LX + f
+  float W2[NH * LH];
+  float W3[NH * LH];
+  float b[3][NH];
+  float w4[NH];
+  float mean[NF], sd[NF];
+  float X[TA * LX];     // normalised features of the tile's atoms (zero rows past the species' end)
+  float Z[3][TA * LH];  // z1, z2, z3 of the tile; the backward pass overwrites them with G1, G2, G3
+  float g[TA];          // dL/dq_raw of the tile's atoms (zero past the end)
+};
+
+__device__ void stage_weights(MlpSmem& sm, const float* __restrict__ P, int c) {
   const float* W = P + c * PS;
-  float x[NF];
-#pragma unroll
-  for (int f = 0; f < NF; f++)
-    x[f] = (feat[(size_t)a * NF + f] - P[OMEAN + c * NF + f]) / P[OSD + c * NF + f];
-  float h[NH];
-  for (int o = 0; o < NH; o++) {
-    float s = W[OB1 + o];
-#pragma unroll
-    for (int f = 0; f < NF; f++) s = fmaf(W[OW1 + o * NF + f], x[f], s);
-    z1[(size_t)a * NH + o] = s;
+  for (int e = threadIdx.x; e < NH * NF; e += 256) sm.W1[(e / NF) * LX + e % NF] = W[OW1 + e];
+  for (int e = threadIdx.x; e < NH * NH; e += 256) {
+    sm.W2[(e / NH) * LH + e % NH] = W[OW2 + e];
+    sm.W3[(e / NH) * LH + e % NH] = W[OW3 + e];
   }
-#pragma unroll
-  for (int f = 0; f < NH; f++) h[f] = silu(z1[(size_t)a * NH + f]);
-  for (int o = 0; o < NH; o++) {
-    float s = W[OB2 + o];
-#pragma unroll
-    for (int f = 0; f < NH; f++) s = fmaf(W[OW2 + o * NH + f], h[f], s);
-    z2[(size_t)a * NH + o] = s;
+  for (int e = threadIdx.x; e < NH; e += 256) {
+    sm.b[0][e] = W[OB1 + e];
+    sm.b[1][e] = W[OB2 + e];
+    sm.b[2][e] = W[OB3 + e];
+    sm.w4[e] = W[OW4 + e];
   }
-#pragma unroll
-  for (int f = 0; f < NH; f++) h[f] = silu(z2[(size_t)a * NH + f]);
-  for (int o = 0; o < NH; o++) {
-    float s = W[OB3 + o];
-#pragma unroll
-    for (int f = 0; f < NH; f++) s = fmaf(W[OW3 + o * NH + f], h[f], s);
-    z3[(size_t)a * NH + o] = s;
+  for (int e = threadIdx.x; e < NF; e += 256) {
+    sm.mean[e] = P[OMEAN + c * NF + e];
+    sm.sd[e] = P[OSD + c * NF + e];
   }
-  float y = W[OB4];
+  __syncthreads();  // the tile loads read mean and sd
+}
+
+// quadrant (wr, wc) of in[TA x K] W[NH x K]^T (silu applied to `in` on load)
+template <int K, bool SILU_IN>
+__device__ __forceinline__ acc16 layer_quadrant(const float* in, int li, const float* W, int lw, int wr, int wc,
+                                                int lane) {
+  acc16 acc;
 #pragma unroll
-  for (int f = 0; f < NH; f++) y = fmaf(W[OW4 + f], silu(z3[(size_t)a * NH + f]), y);
-  qraw[a] = (double)P[OSHIFT + c] + (double)y;
+  for (int e = 0; e < 16; e++) acc[e] = 0.f;
+  const int m = wr * 32 + (lane & 31), n = wc * 32 + (lane & 31), kq = lane >> 5;
+#pragma unroll 8
+  for (int k0 = 0; k0 < K; k0 += 2) {
+    float a = in[m * li + k0 + kq];
+    if (SILU_IN) a = silu(a);
+    acc = mma32(a, W[n * lw + k0 + kq], acc);
+  }
+  return acc;
+}
+
+__device__ __forceinline__ void store_quadrant(float* out, const float* bias, acc16 acc, int wr, int wc, int lane) {
+  const int col = wc * 32 + (lane & 31);
+#pragma unroll
+  for (int e = 0; e < 16; e++) out[(wr * 32 + acc_row(lane, e)) * LH + col] = acc[e] + bias[col];
+}
+
+// load tile t of species list idx (features normalised, g of the atoms when
+// BWD), then z1, z2, z3 into LDS
+template <bool BWD>
+__device__ void tile_forward(MlpSmem& sm, const int* __restrict__ idx, int nidx, int t,
+                             const float* __restrict__ feat, const double* __restrict__ gq, int wave, int lane) {
+  const int wr = wave >> 1, wc = wave & 1;
+  for (int e = threadIdx.x; e < TA * NF; e += 256) {
+    const int r = e / NF, f = e - r * NF, k = t * TA + r;
+    sm.X[r * LX + f] = k < nidx ? (feat[(size_t)idx[k] * NF + f] - sm.mean[f]) / sm.sd[f] : 0.f;
+  }
+  if (BWD && threadIdx.x < TA) {
+    const int k = t * TA + threadIdx.x;
+    sm.g[threadIdx.x] = k < nidx ? (float)gq[idx[k]] : 0.f;
+  }
+  __syncthreads();
+  store_quadrant(sm.Z[0], sm.b[0], layer_quadrant<NF, false>(sm.X, LX, sm.W1, LX, wr, wc, lane), wr, wc, lane);
+  __syncthreads();
+  store_quadrant(sm.Z[1], sm.b[1], layer_quadrant<NH, true>(sm.Z[0], LH, sm.W2, LH, wr, wc, lane), wr, wc, lane);
+  __syncthreads();
+  store_quadrant(sm.Z[2], sm.b[2], layer_quadrant<NH, true>(sm.Z[1], LH, sm.W3, LH, wr, wc, lane), wr, wc, lane);
+  __syncthreads();
+}
+
+// q_raw of every atom of species c (idx: its atoms, nidx of them)
+__global__ __launch_bounds__(256) void k_mlp_forward(int c, const int* __restrict__ idx, int nidx,
+                                                     const float* __restrict__ feat, const float* __restrict__ P,
+                                                     double* __restrict__ qraw) {
+  __shared__ MlpSmem sm;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int ntile = (nidx + TA - 1) / TA;
+  if ((int)blockIdx.x >= ntile) return;
+  stage_weights(sm, P, c);
+  const float b4 = P[c * PS + OB4], shift = P[OSHIFT + c];
+  for (int t = blockIdx.x; t < ntile; t += gridDim.x) {
+    tile_forward<false>(sm, idx, nidx, t, feat, nullptr, wave, lane);
+    if (threadIdx.x < TA && t * TA + (int)threadIdx.x < nidx) {
+      const float* z3 = sm.Z[2] + threadIdx.x * LH;
+      float y = b4;
+#pragma unroll 8
+      for (int f = 0; f < NH; f++) y = fmaf(sm.w4[f], silu(z3[f]), y);
+      qraw[idx[t * TA + threadIdx.x]] = (double)shift + (double)y;
+    }
+    __syncthreads();  // the next tile overwrites X and Z
+  }
+}
+
+// dL/dW of species c over its atoms: workgroup p adds the tiles p, p + NPB, ...
+// and writes its sums to partial[p][c PS ...] (zeros when it has no tile)
+__global__ __launch_bounds__(256) void k_mlp_backward(int c, const int* __restrict__ idx, int nidx,
+                                                      const float* __restrict__ feat, const float* __restrict__ P,
+                                                      const double* __restrict__ gq, float* __restrict__ partial) {
+  __shared__ MlpSmem sm;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int wr = wave >> 1, wc = wave & 1, tid = threadIdx.x;
+  const int ntile = (nidx + TA - 1) / TA;
+  stage_weights(sm, P, c);
+  acc16 dw3, dw2, dw1a, dw1b;  // dW3, dW2 quadrant (wr, wc); dW1 tiles q = wave and q = wave + 4 (waves 0, 1)
+#pragma unroll
+  for (int e = 0; e < 16; e++) dw3[e] = dw2[e] = dw1a[e] = dw1b[e] = 0.f;
+  float dvec = 0.f;  // tid 0-63 db1, 64-127 db2, 128-191 db3, 192-255 dw4
+  float db4 = 0.f;   // tid 0
+  const int m32 = lane & 31, kq = lane >> 5;
+  // dW1 tile q: rows (outputs) 32 (q / 3), columns (features) 32 (q % 3), features >= NF are zero
+  const int qa = wave, qb = wave + 4;
+  for (int t = blockIdx.x; t < ntile; t += gridDim.x) {
+    tile_forward<true>(sm, idx, nidx, t, feat, gq, wave, lane);
+    // layer 4: dw4 = sum_r g_r silu(z3), db4 = sum_r g_r
+    if (tid >= 192) {
+      const int o = tid - 192;
+      float s4 = 0.f;
+      for (int r = 0; r < TA; r++) s4 = fmaf(sm.g[r], silu(sm.Z[2][r * LH + o]), s4);
+      dvec += s4;
+    }
+    if (tid == 0) {
+      float s4 = 0.f;
+      for (int r = 0; r < TA; r++) s4 += sm.g[r];
+      db4 += s4;
+    }
+    __syncthreads();
+    // G3 = g w4 silu'(z3), in place of z3
+    for (int e = tid; e < TA * NH; e += 256) {
+      const int r = e / NH, o = e - r * NH;
+      float* z = &sm.Z[2][r * LH + o];
+      *z = sm.g[r] * sm.w4[o] * dsilu(*z);
+    }
+    __syncthreads();
+    // dW3 += G3^T silu(z2); P2 = G3 W3; db3 += column sums of G3
+    {
+      acc16 p2;
+#pragma unroll
+      for (int e = 0; e < 16; e++) p2[e] = 0.f;
+#pragma unroll 8
+      for (int k0 = 0; k0 < TA; k0 += 2) {
+        const int r = k0 + kq;
+        dw3 = mma32(sm.Z[2][r * LH + wr * 32 + m32], silu(sm.Z[1][r * LH + wc * 32 + m32]), dw3);
+      }
+#pragma unroll 8
+      for (int k0 = 0; k0 < NH; k0 += 2) {
+        const int o = k0 + kq;
+        p2 = mma32(sm.Z[2][(wr * 32 + m32) * LH + o], sm.W3[o * LH + wc * 32 + m32], p2);
+      }
+      if (tid >= 128 && tid < 192) {
+        const int o = tid - 128;
+        float s3 = 0.f;
+        for (int r = 0; r < TA; r++) s3 += sm.Z[2][r * LH + o];
+        dvec += s3;
+      }
+      __syncthreads();
+      // G2 = P2 silu'(z2), in place of z2 (each lane its own elements)
+      const int col = wc * 32 + m32;
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        float* z = &sm.Z[1][(wr * 32 + acc_row(lane, e)) * LH + col];
+        *z = p2[e] * dsilu(*z);
+      }
+      __syncthreads();
+    }
+    // dW2 += G2^T silu(z1); P1 = G2 W2; db2 += column sums of G2
+    {
+      acc16 p1;
+#pragma unroll
+      for (int e = 0; e < 16; e++) p1[e] = 0.f;
+#pragma unroll 8
+      for (int k0 = 0; k0 < TA; k0 += 2) {
+        const int r = k0 + kq;
+        dw2 = mma32(sm.Z[1][r * LH + wr * 32 + m32], silu(sm.Z[0][r * LH + wc * 32 + m32]), dw2);
+      }
+#pragma unroll 8
+      for (int k0 = 0; k0 < NH; k0 += 2) {
+        const int o = k0 + kq;
+        p1 = mma32(sm.Z[1][(wr * 32 + m32) * LH + o], sm.W2[o * LH + wc * 32 + m32], p1);
+      }
+      if (tid >= 64 && tid < 128) {
+        const int o = tid - 64;
+        float s2 = 0.f;
+        for (int r = 0; r < TA; r++) s2 += sm.Z[1][r * LH + o];
+        dvec += s2;
+      }
+      __syncthreads();
+      const int col = wc * 32 + m32;
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        float* z = &sm.Z[0][(wr * 32 + acc_row(lane, e)) * LH + col];
+        *z = p1[e] * dsilu(*z);
+      }
+      __syncthreads();
+    }
+    // dW1 += G1^T X; db1 += column sums of G1
+#pragma unroll 8
+    for (int k0 = 0; k0 < TA; k0 += 2) {
+      const int r = k0 + kq;
+      const float* g1 = sm.Z[0] + r * LH;
+      const float* x = sm.X + r * LX;
+      const int fa = 32 * (qa % 3) + m32;
+      dw1a = mma32(g1[32 * (qa / 3) + m32], fa < NF ? x[fa] : 0.f, dw1a);
+      if (wave < 2) {
+        const int fb = 32 * (qb % 3) + m32;
+        dw1b = mma32(g1[32 * (qb / 3) + m32], fb < NF ? x[fb] : 0.f, dw1b);
+      }
+    }
+    if (tid < 64) {
+      float s1 = 0.f;
+      for (int r = 0; r < TA; r++) s1 += sm.Z[0][r * LH + tid];
+      dvec += s1;
+    }
+    __syncthreads();  // the next tile overwrites X, Z and g
+  }
+  float* out = partial + (size_t)blockIdx.x * NTRAIN + c * PS;
+  {
+    const int col = wc * 32 + m32;
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+      const int row = wr * 32 + acc_row(lane, e);
+      out[OW3 + row * NH + col] = dw3[e];
+      out[OW2 + row * NH + col] = dw2[e];
+    }
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+      const int f = 32 * (qa % 3) + m32;
+      if (f < NF) out[OW1 + (32 * (qa / 3) + acc_row(lane, e)) * NF + f] = dw1a[e];
+    }
+    if (wave < 2) {
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        const int f = 32 * (qb % 3) + m32;
+        if (f < NF) out[OW1 + (32 * (qb / 3) + acc_row(lane, e)) * NF + f] = dw1b[e];
+      }
+    }
+  }
+  const int off = tid < 64 ? OB1 : (tid < 128 ? OB2 : (tid < 192 ? OB3 : OW4));
+  out[off + (tid & 63)] = dvec;
+  if (tid == 0) out[OB4] = db4;
 }
 
 // mean projection per structure: q = q_raw - (sum q_raw - Q) / n_seg
@@ -198,81 +429,11 @@ __global__ void k_sum(const double* __restrict__ v, int n, double* __restrict__ 
   *out = s;
 }
 
-__global__ __launch_bounds__(128) void k_mlp_backward(int n, const int* __restrict__ Z, const float* __restrict__ P,
-                               const float* __restrict__ z1, const float* __restrict__ z2,
-                               const float* __restrict__ z3, const double* __restrict__ g,
-                               float* __restrict__ d1, float* __restrict__ d2,
-                               float* __restrict__ d3, float* __restrict__ d4) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= n) return;
-  const float* W = P + species(Z[a]) * PS;
-  const float dy = (float)g[a];
-  d4[a] = dy;
-  float dz[NH];
-#pragma unroll
-  for (int o = 0; o < NH; o++) dz[o] = dy * W[OW4 + o] * dsilu(z3[(size_t)a * NH + o]);
-#pragma unroll
-  for (int o = 0; o < NH; o++) d3[(size_t)a * NH + o] = dz[o];
-  // d2 = (W3^T dz3) * silu'(z2)
-  for (int f = 0; f < NH; f++) {
-    float s = 0;
-#pragma unroll
-    for (int o = 0; o < NH; o++) s = fmaf(W[OW3 + o * NH + f], dz[o], s);
-    d2[(size_t)a * NH + f] = s * dsilu(z2[(size_t)a * NH + f]);
-  }
-#pragma unroll
-  for (int o = 0; o < NH; o++) dz[o] = d2[(size_t)a * NH + o];
-  for (int f = 0; f < NH; f++) {
-    float s = 0;
-#pragma unroll
-    for (int o = 0; o < NH; o++) s = fmaf(W[OW2 + o * NH + f], dz[o], s);
-    d1[(size_t)a * NH + f] = s * dsilu(z1[(size_t)a * NH + f]);
-  }
-}
-
-// partial[p][...] = sum over atoms of species c in partition p of D[a][o] * Hin[a][i]
-// layer 1: Hin = normalized features (NF); 2: silu(z1); 3: silu(z2); 4: silu(z3), D = d4
-__global__ __launch_bounds__(128) void k_wgrad(int layer, int c, const int* __restrict__ idx, int nidx,
-                                               const float* __restrict__ feat,
-                                               const float* __restrict__ P,
-                                               const float* __restrict__ hz,  // z of the input layer
-                                               const float* __restrict__ D, float* __restrict__ partial) {
-  const int o = blockIdx.x, p = blockIdx.y, i = threadIdx.x;
-  const int nin = layer == 1 ? NF : NH;
-  const int nout = layer == 4 ? 1 : NH;
-  if (i > nin) return;
-  const int per = (nidx + NPART - 1) / NPART;
-  const int ab = p * per, ae = min(nidx, ab + per);
-  float acc = 0;
-  for (int t = ab; t < ae; t++) {
-    const int a = idx[t];
-    const float d = D[(size_t)a * nout + o];
-    float h;
-    if (i == nin)
-      h = 1.0f;  // bias
-    else if (layer == 1)
-      h = (feat[(size_t)a * NF + i] - P[OMEAN + c * NF + i]) / P[OSD + c * NF + i];
-    else
-      h = silu(hz[(size_t)a * NH + i]);
-    acc = fmaf(d, h, acc);
-  }
-  int woff, boff;
-  if (layer == 1) woff = OW1, boff = OB1;
-  else if (layer == 2) woff = OW2, boff = OB2;
-  else if (layer == 3) woff = OW3, boff = OB3;
-  else woff = OW4, boff = OB4;
-  float* out = partial + (size_t)p * NTRAIN + c * PS;
-  if (i == nin)
-    out[boff + o] = acc;
-  else
-    out[woff + o * nin + i] = acc;
-}
-
 __global__ void k_reduce_partials(const float* __restrict__ partial, float* __restrict__ grad) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= NTRAIN) return;
   float s = 0;
-  for (int p = 0; p < NPART; p++) s += partial[(size_t)p * NTRAIN + j];
+  for (int p = 0; p < NPB; p++) s += partial[(size_t)p * NTRAIN + j];
   grad[j] = s;
 }
 
@@ -296,8 +457,7 @@ struct ltb_mlp {
   float *d_P = nullptr, *d_grad = nullptr, *d_m = nullptr, *d_v = nullptr, *d_partial = nullptr;
   int adam_t = 0;
   int n = 0, nseg = 0;
-  float *d_feat = nullptr, *d_z1 = nullptr, *d_z2 = nullptr, *d_z3 = nullptr;
-  float *d_d1 = nullptr, *d_d2 = nullptr, *d_d3 = nullptr, *d_d4 = nullptr;
+  float* d_feat = nullptr;
   int *d_Z = nullptr, *d_seg = nullptr, *d_idx[2] = {nullptr, nullptr};
   int nidx[2] = {0, 0};
   double *d_qtot = nullptr, *d_t = nullptr, *d_qraw = nullptr, *d_q = nullptr, *d_g = nullptr,
@@ -308,12 +468,12 @@ struct ltb_mlp {
 namespace {
 
 void free_data(ltb_mlp* m) {
-  void* ps[] = {m->d_feat, m->d_z1, m->d_z2, m->d_z3, m->d_d1, m->d_d2, m->d_d3, m->d_d4, m->d_Z,
+  void* ps[] = {m->d_feat, m->d_Z,
                 m->d_seg, m->d_idx[0], m->d_idx[1], m->d_qtot, m->d_t, m->d_qraw, m->d_q, m->d_g,
                 m->d_lpart};
   for (void* p : ps)
     if (p) (void)hipFree(p);
-  m->d_feat = m->d_z1 = m->d_z2 = m->d_z3 = m->d_d1 = m->d_d2 = m->d_d3 = m->d_d4 = nullptr;
+  m->d_feat = nullptr;
   m->d_Z = m->d_seg = m->d_idx[0] = m->d_idx[1] = nullptr;
   m->d_qtot = m->d_t = m->d_qraw = m->d_q = m->d_g = m->d_lpart = nullptr;
   m->n = m->nseg = 0;
@@ -359,11 +519,19 @@ int descriptors_device(ltb_ctx* c, const ltb_system* s, float* d_feat) {
   return LTB_OK;
 }
 
+// forward tiles: up to 4 workgroups per CU's worth of tiles in flight
+int forward_grid(const ltb_mlp* m, int nidx) {
+  const int ntile = (nidx + TA - 1) / TA;
+  return std::max(1, std::min(ntile, 4 * std::max(m->ctx->cu_count, 1)));
+}
+
 int forward_async(ltb_mlp* m) {
   ltb_ctx* c = m->ctx;
-  const int g = (m->n + 127) / 128;
-  hipLaunchKernelGGL(k_mlp_forward, dim3(g), dim3(128), 0, c->stream, m->n, m->d_feat, m->d_Z, m->d_P,
-                     m->d_z1, m->d_z2, m->d_z3, m->d_qraw);
+  for (int sp = 0; sp < 2; sp++) {
+    if (m->nidx[sp] == 0) continue;
+    hipLaunchKernelGGL(k_mlp_forward, dim3(forward_grid(m, m->nidx[sp])), dim3(256), 0, c->stream, sp,
+                       m->d_idx[sp], m->nidx[sp], m->d_feat, m->d_P, m->d_qraw);
+  }
   hipLaunchKernelGGL(k_project, dim3(m->nseg), dim3(256), 0, c->stream, m->d_seg, m->d_qtot, m->d_qraw,
                      m->d_q);
   LTB_HIP(c, hipGetLastError());
@@ -375,20 +543,10 @@ int loss_grad_async(ltb_mlp* m) {
   hipLaunchKernelGGL(k_loss_grad, dim3(m->nseg), dim3(256), 0, c->stream, m->d_seg, m->d_q, m->d_t,
                      m->n, m->d_g, m->d_lpart);
   hipLaunchKernelGGL(k_sum, dim3(1), dim3(64), 0, c->stream, m->d_lpart, m->nseg, m->d_loss);
-  const int g = (m->n + 127) / 128;
-  hipLaunchKernelGGL(k_mlp_backward, dim3(g), dim3(128), 0, c->stream, m->n, m->d_Z, m->d_P, m->d_z1,
-                     m->d_z2, m->d_z3, m->d_g, m->d_d1, m->d_d2, m->d_d3, m->d_d4);
-  for (int sp = 0; sp < 2; sp++) {
-    if (m->nidx[sp] == 0) continue;
-    hipLaunchKernelGGL(k_wgrad, dim3(NH, NPART), dim3(128), 0, c->stream, 1, sp, m->d_idx[sp],
-                       m->nidx[sp], m->d_feat, m->d_P, (const float*)nullptr, m->d_d1, m->d_partial);
-    hipLaunchKernelGGL(k_wgrad, dim3(NH, NPART), dim3(128), 0, c->stream, 2, sp, m->d_idx[sp],
-                       m->nidx[sp], m->d_feat, m->d_P, m->d_z1, m->d_d2, m->d_partial);
-    hipLaunchKernelGGL(k_wgrad, dim3(NH, NPART), dim3(128), 0, c->stream, 3, sp, m->d_idx[sp],
-                       m->nidx[sp], m->d_feat, m->d_P, m->d_z2, m->d_d3, m->d_partial);
-    hipLaunchKernelGGL(k_wgrad, dim3(1, NPART), dim3(128), 0, c->stream, 4, sp, m->d_idx[sp],
-                       m->nidx[sp], m->d_feat, m->d_P, m->d_z3, m->d_d4, m->d_partial);
-  }
+  // both species always run: a species without atoms writes zero partials
+  for (int sp = 0; sp < 2; sp++)
+    hipLaunchKernelGGL(k_mlp_backward, dim3(NPB), dim3(256), 0, c->stream, sp, m->d_idx[sp], m->nidx[sp],
+                       m->d_feat, m->d_P, m->d_g, m->d_partial);
   hipLaunchKernelGGL(k_reduce_partials, dim3((NTRAIN + 255) / 256), dim3(256), 0, c->stream,
                      m->d_partial, m->d_grad);
   LTB_HIP(c, hipGetLastError());
@@ -429,7 +587,7 @@ int ltb_mlp_create(ltb_ctx* c, unsigned long long seed, ltb_mlp** out) {
       hipMalloc(&m->d_grad, sizeof(float) * NTRAIN) != hipSuccess ||
       hipMalloc(&m->d_m, sizeof(float) * NTRAIN) != hipSuccess ||
       hipMalloc(&m->d_v, sizeof(float) * NTRAIN) != hipSuccess ||
-      hipMalloc(&m->d_partial, sizeof(float) * NTRAIN * NPART) != hipSuccess ||
+      hipMalloc(&m->d_partial, sizeof(float) * NTRAIN * NPB) != hipSuccess ||
       hipMalloc(&m->d_loss, sizeof(double)) != hipSuccess) {
     free_params(m);
     delete m;
@@ -438,7 +596,7 @@ int ltb_mlp_create(ltb_ctx* c, unsigned long long seed, ltb_mlp** out) {
   if (hipMemcpy(m->d_P, P.data(), sizeof(float) * NPARAM, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(m->d_m, 0, sizeof(float) * NTRAIN) != hipSuccess ||
       hipMemset(m->d_v, 0, sizeof(float) * NTRAIN) != hipSuccess ||
-      hipMemset(m->d_partial, 0, sizeof(float) * NTRAIN * NPART) != hipSuccess) {
+      hipMemset(m->d_partial, 0, sizeof(float) * NTRAIN * NPB) != hipSuccess) {
     free_params(m);
     delete m;
     return set_err(c, LTB_ERR_DEVICE, "MLP parameter initialisation failed");
@@ -514,10 +672,7 @@ namespace {
 int upload_data(ltb_mlp* m, int n, const float* feat, const int* Z, int nseg, const int* seg, const double* qtot,
                 const double* target, const std::vector<int> (&idx)[2]) {
   ltb_ctx* c = m->ctx;
-  const size_t act = sizeof(float) * NH * (size_t)n;
   LTB_HIP(c, hipMalloc(&m->d_feat, sizeof(float) * NF * (size_t)n));
-  for (float** p : {&m->d_z1, &m->d_z2, &m->d_z3, &m->d_d1, &m->d_d2, &m->d_d3}) LTB_HIP(c, hipMalloc(p, act));
-  LTB_HIP(c, hipMalloc(&m->d_d4, sizeof(float) * n));
   LTB_HIP(c, hipMalloc(&m->d_Z, sizeof(int) * n));
   LTB_HIP(c, hipMalloc(&m->d_seg, sizeof(int) * (nseg + 1)));
   for (int sp = 0; sp < 2; sp++) {

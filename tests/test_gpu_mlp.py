@@ -53,6 +53,41 @@ def test_forward_loss_and_gradient_match_oracle(ctx):
     m.close()
 
 
+def test_gradient_over_many_tiles_matches_oracle(ctx):
+    """A 12x12x12 and an 11x12x12 cell, 29,808 atoms: 19,872 O atoms are 311
+    tiles of 64, more than the 256 workgroups of the backward pass, so
+    workgroups add several tiles; both species end in a ragged tile.  Forward,
+    loss and gradient against the oracle."""
+    feats, Zs, seg, qtot, tgt = [], [], [0], [], []
+    rng = np.random.default_rng(5)
+    for cells, seed in (((12, 12, 12), 21), ((11, 12, 12), 22)):
+        s = L.System.quartz(*cells, sigma_ang=0.1, seed=seed)
+        Z, _ = s.coords()
+        feats.append(ctx.descriptors(s))
+        Zs.append(Z)
+        seg.append(seg[-1] + s.nat)
+        qtot.append(s.nelectrons)
+        tgt.append(np.where(Z == 14, 2.6, 6.7) + 0.02 * rng.standard_normal(s.nat))
+    feat, Z, t = np.concatenate(feats), np.concatenate(Zs), np.concatenate(tgt)
+    seg, qtot = np.array(seg, np.int32), np.array(qtot)
+    assert (Z == 8).sum() // 64 > 256 and (Z == 8).sum() % 64 != 0 and (Z == 14).sum() % 64 != 0
+    m = L.Mlp(ctx, seed=3)
+    P = random_params(4, feat.astype(float), Z).astype(np.float32)
+    m.set_params(P)
+    m.set_data(feat, Z, seg, qtot, t)
+    q = m.forward()
+    qref = OM.forward(P.astype(float), feat.astype(float), Z, seg, qtot)
+    assert np.abs(q - qref).max() < 2e-4
+    loss, g = m.loss_grad()
+    lref, gref = OM.loss_grad(P.astype(float), feat.astype(float), Z, seg, qtot, t)
+    assert loss == pytest.approx(lref, rel=1e-4)
+    assert np.abs(g - gref).max() < 2e-4 * np.abs(gref).max()
+    # the same data again gives the same gradient bit for bit (fixed-order sums)
+    _, g2 = m.loss_grad()
+    assert np.array_equal(g, g2)
+    m.close()
+
+
 def test_adam_training_reduces_loss(ctx):
     feat, Z, seg, qtot, t = dataset(ctx)
     m = L.Mlp(ctx, seed=11)
