@@ -384,6 +384,51 @@ def sp2_measure(args):
     ctx.close()
 
 
+def mlp_measure(args):
+    """Training-step time of the learned-guess network (MFMA forward and
+    backward) on the descriptors of 256 uneven cells (2x2x2 .. 5x5x4 quartz,
+    86,274 atoms) and of the same set repeated 8 times (690,192 atoms).
+    Targets are synthetic (per-species constants plus noise): only the time is
+    measured.  MFMA flops per atom and Adam step: forward 2 (80 64 + 2 64^2),
+    backward with the forward recomputed 2 (80 64 + 2 64^2) + 2 (2 64^2) (layer
+    gradients) + 2 (96 64 + 2 64^2) (weight gradients, features padded to 96)."""
+    import numpy as np
+    import largetb_amd as L
+    from largetb_amd.learned import batch_structures
+    ctx = L.Context(0)
+    feats, Zs, seg, qtot = [], [], [0], []
+    for l, m_, n, sg, sd in batch_structures(256, 2026):
+        s = L.System.quartz(l, m_, n, sigma_ang=sg, seed=sd)
+        Z, _ = s.coords()
+        feats.append(ctx.descriptors(s))
+        Zs.append(Z)
+        seg.append(seg[-1] + s.nat)
+        qtot.append(s.nelectrons)
+    feat, Z = np.concatenate(feats), np.concatenate(Zs)
+    flops_atom = 2 * (80 * 64 + 2 * 64 * 64) * 2 + 2 * (2 * 64 * 64) + 2 * (96 * 64 + 2 * 64 * 64)
+    rng = np.random.default_rng(args.seed)
+    out = {"workload": "learned-guess MLP training steps (80-64-64-64-1 per species, full batch)",
+           "mfma_flops_per_atom_step": flops_atom}
+    for rep in (1, 8):
+        f = np.tile(feat, (rep, 1))
+        z = np.tile(Z, rep)
+        sgs = np.concatenate([[0]] + [np.array(seg[1:]) + k * seg[-1] for k in range(rep)]).astype(np.int32)
+        t = np.where(z == 14, 2.6, 6.7) + 0.02 * rng.standard_normal(len(z))
+        m = L.Mlp(ctx, seed=1)
+        m.set_data(f, z, sgs, np.tile(qtot, rep), t)
+        m.fit_normalization()
+        m.adam(5, lr=1e-3)
+        t0 = time.perf_counter()
+        m.adam(args.steps * 10, lr=1e-3)
+        dt = (time.perf_counter() - t0) / (args.steps * 10)
+        out[f"atoms_{len(z)}"] = {"ms_per_adam_step": round(dt * 1e3, 3),
+                                  "tflops_mfma": round(flops_atom * len(z) / dt / 1e12, 2)}
+        m.close()
+    out["command"] = "python3 -u bench.py --mlp"
+    print(json.dumps(out))
+    ctx.close()
+
+
 def batch_measure(args):
     """Many independent uneven cells in one batched SCC (2x2x2 .. 5x5x4 quartz,
     sigma 0.02-0.15 A from the seed, M=256), FP32 bulk with FP64 finish, every
@@ -588,6 +633,8 @@ def main():
     ap.add_argument("--max-iter", type=int, default=400)
     ap.add_argument("--compare", nargs=2, metavar=("PLAIN", "FAST"),
                     help="errors and speed-up between two finished segment state files")
+    ap.add_argument("--mlp", action="store_true",
+                    help="learned-guess network: training-step time of the MFMA kernels")
     ap.add_argument("--assembly", action="store_true",
                     help="host vs device neighbour list + H assembly (48x48x24 cells by default)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -605,6 +652,9 @@ def main():
         return
     if args.assembly:
         assembly_measure(args)
+        return
+    if args.mlp:
+        mlp_measure(args)
         return
     if args.segment:
         scc_segment(args)

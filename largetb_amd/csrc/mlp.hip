@@ -51,9 +51,11 @@ constexpr int NPB = 256;   // persistent workgroups of the backward pass (partia
 constexpr int TA = 64;     // atoms per tile
 constexpr int LX = NF + 1, LH = NH + 1;  // odd LDS row strides: row and column reads are conflict-free
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// the logistic function through the hardware reciprocal (about 1 ulp)
+__device__ __forceinline__ float sigm(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigm(z); }
 __device__ __forceinline__ float dsilu(float z) {
-  float s = 1.0f / (1.0f + __expf(-z));
+  const float s = sigm(z);
   return s * (1.0f + z * (1.0f - s));
 }
 __device__ __forceinline__ int species(int Z) { return Z == 14 ? 0 : 1; }
@@ -180,20 +182,40 @@ __device__ __forceinline__ void store_quadrant(float* out, const float* bias, ac
   for (int e = 0; e < 16; e++) out[(wr * 32 + acc_row(lane, e)) * LH + col] = acc[e] + bias[col];
 }
 
-// load tile t of species list idx (features normalised, g of the atoms when
-// BWD), then z1, z2, z3 into LDS
+// one tile's raw features (and g) in registers: thread i holds elements
+// i, i + 256, ... of the tile's 64 x 80 block, so the next tile's gathers are in
+// flight while the current tile computes
+constexpr int NXR = TA * NF / 256;
+static_assert(TA * NF % 256 == 0, "whole feature block per thread");
+struct TileRegs {
+  float x[NXR];
+  float g;
+};
+
 template <bool BWD>
-__device__ void tile_forward(MlpSmem& sm, const int* __restrict__ idx, int nidx, int t,
-                             const float* __restrict__ feat, const double* __restrict__ gq, int wave, int lane) {
-  const int wr = wave >> 1, wc = wave & 1;
-  for (int e = threadIdx.x; e < TA * NF; e += 256) {
-    const int r = e / NF, f = e - r * NF, k = t * TA + r;
-    sm.X[r * LX + f] = k < nidx ? (feat[(size_t)idx[k] * NF + f] - sm.mean[f]) / sm.sd[f] : 0.f;
+__device__ __forceinline__ void tile_fetch(TileRegs& tr, const int* __restrict__ idx, int nidx, int t, int ntile,
+                                           const float* __restrict__ feat, const double* __restrict__ gq) {
+#pragma unroll
+  for (int j = 0; j < NXR; j++) {
+    const int e = threadIdx.x + 256 * j, r = e / NF, f = e - r * NF, k = t * TA + r;
+    tr.x[j] = (t < ntile && k < nidx) ? feat[(size_t)idx[k] * NF + f] : 0.f;
   }
-  if (BWD && threadIdx.x < TA) {
+  if (BWD) {
     const int k = t * TA + threadIdx.x;
-    sm.g[threadIdx.x] = k < nidx ? (float)gq[idx[k]] : 0.f;
+    tr.g = (threadIdx.x < TA && t < ntile && k < nidx) ? (float)gq[idx[k]] : 0.f;
   }
+}
+
+// the tile's normalised features (and g) into LDS, then z1, z2, z3
+template <bool BWD>
+__device__ void tile_forward(MlpSmem& sm, const TileRegs& tr, int nidx, int t, int wave, int lane) {
+  const int wr = wave >> 1, wc = wave & 1;
+#pragma unroll
+  for (int j = 0; j < NXR; j++) {
+    const int e = threadIdx.x + 256 * j, r = e / NF, f = e - r * NF;
+    sm.X[r * LX + f] = t * TA + r < nidx ? (tr.x[j] - sm.mean[f]) / sm.sd[f] : 0.f;
+  }
+  if (BWD && threadIdx.x < TA) sm.g[threadIdx.x] = tr.g;
   __syncthreads();
   store_quadrant(sm.Z[0], sm.b[0], layer_quadrant<NF, false>(sm.X, LX, sm.W1, LX, wr, wc, lane), wr, wc, lane);
   __syncthreads();
@@ -201,6 +223,27 @@ __device__ void tile_forward(MlpSmem& sm, const int* __restrict__ idx, int nidx,
   __syncthreads();
   store_quadrant(sm.Z[2], sm.b[2], layer_quadrant<NH, true>(sm.Z[1], LH, sm.W3, LH, wr, wc, lane), wr, wc, lane);
   __syncthreads();
+}
+
+// sum over the 64 lanes of a wave (fixed order; every lane gets the sum)
+__device__ __forceinline__ float wave_sum(float v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// column sums over the tile's 64 rows of M (LDS, row stride LH) for the 16
+// columns 16 wave .. 16 wave + 15: lane l adds rows 16 (l / 16) .. + 15 of column
+// 16 wave + l % 16, then the four row groups are added across lanes; every lane
+// returns the sum of its column.  G_WEIGHT: sum_r g_r silu(M[r][o]) instead.
+template <bool G_WEIGHT>
+__device__ __forceinline__ float colsum16(const MlpSmem& sm, const float* M, int wave, int lane) {
+  const int col = 16 * wave + (lane & 15), r0 = 16 * (lane >> 4);
+  float s = 0.f;
+#pragma unroll
+  for (int r = r0; r < r0 + 16; r++) s = G_WEIGHT ? fmaf(sm.g[r], silu(M[r * LH + col]), s) : s + M[r * LH + col];
+  s += __shfl_xor(s, 16);
+  s += __shfl_xor(s, 32);
+  return s;
 }
 
 // q_raw of every atom of species c (idx: its atoms, nidx of them)
@@ -211,10 +254,13 @@ __global__ __launch_bounds__(256) void k_mlp_forward(int c, const int* __restric
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int ntile = (nidx + TA - 1) / TA;
   if ((int)blockIdx.x >= ntile) return;
+  TileRegs tr;
+  tile_fetch<false>(tr, idx, nidx, blockIdx.x, ntile, feat, nullptr);
   stage_weights(sm, P, c);
   const float b4 = P[c * PS + OB4], shift = P[OSHIFT + c];
   for (int t = blockIdx.x; t < ntile; t += gridDim.x) {
-    tile_forward<false>(sm, idx, nidx, t, feat, nullptr, wave, lane);
+    tile_forward<false>(sm, tr, nidx, t, wave, lane);
+    tile_fetch<false>(tr, idx, nidx, t + gridDim.x, ntile, feat, nullptr);  // the next tile
     if (threadIdx.x < TA && t * TA + (int)threadIdx.x < nidx) {
       const float* z3 = sm.Z[2] + threadIdx.x * LH;
       float y = b4;
@@ -235,29 +281,23 @@ __global__ __launch_bounds__(256) void k_mlp_backward(int c, const int* __restri
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int wr = wave >> 1, wc = wave & 1, tid = threadIdx.x;
   const int ntile = (nidx + TA - 1) / TA;
+  TileRegs tr;
+  tile_fetch<true>(tr, idx, nidx, blockIdx.x, ntile, feat, gq);
   stage_weights(sm, P, c);
   acc16 dw3, dw2, dw1a, dw1b;  // dW3, dW2 quadrant (wr, wc); dW1 tiles q = wave and q = wave + 4 (waves 0, 1)
 #pragma unroll
   for (int e = 0; e < 16; e++) dw3[e] = dw2[e] = dw1a[e] = dw1b[e] = 0.f;
-  float dvec = 0.f;  // tid 0-63 db1, 64-127 db2, 128-191 db3, 192-255 dw4
-  float db4 = 0.f;   // tid 0
+  // bias gradients: lane l of wave w holds those of output 16 w + l % 16
+  float db1 = 0.f, db2 = 0.f, db3 = 0.f, dw4 = 0.f, db4 = 0.f;
   const int m32 = lane & 31, kq = lane >> 5;
   // dW1 tile q: rows (outputs) 32 (q / 3), columns (features) 32 (q % 3), features >= NF are zero
   const int qa = wave, qb = wave + 4;
   for (int t = blockIdx.x; t < ntile; t += gridDim.x) {
-    tile_forward<true>(sm, idx, nidx, t, feat, gq, wave, lane);
+    tile_forward<true>(sm, tr, nidx, t, wave, lane);
+    tile_fetch<true>(tr, idx, nidx, t + gridDim.x, ntile, feat, gq);  // the next tile
     // layer 4: dw4 = sum_r g_r silu(z3), db4 = sum_r g_r
-    if (tid >= 192) {
-      const int o = tid - 192;
-      float s4 = 0.f;
-      for (int r = 0; r < TA; r++) s4 = fmaf(sm.g[r], silu(sm.Z[2][r * LH + o]), s4);
-      dvec += s4;
-    }
-    if (tid == 0) {
-      float s4 = 0.f;
-      for (int r = 0; r < TA; r++) s4 += sm.g[r];
-      db4 += s4;
-    }
+    dw4 += colsum16<true>(sm, sm.Z[2], wave, lane);
+    if (wave == 0) db4 += wave_sum(sm.g[lane]);
     __syncthreads();
     // G3 = g w4 silu'(z3), in place of z3
     for (int e = tid; e < TA * NH; e += 256) {
@@ -281,12 +321,7 @@ __global__ __launch_bounds__(256) void k_mlp_backward(int c, const int* __restri
         const int o = k0 + kq;
         p2 = mma32(sm.Z[2][(wr * 32 + m32) * LH + o], sm.W3[o * LH + wc * 32 + m32], p2);
       }
-      if (tid >= 128 && tid < 192) {
-        const int o = tid - 128;
-        float s3 = 0.f;
-        for (int r = 0; r < TA; r++) s3 += sm.Z[2][r * LH + o];
-        dvec += s3;
-      }
+      db3 += colsum16<false>(sm, sm.Z[2], wave, lane);
       __syncthreads();
       // G2 = P2 silu'(z2), in place of z2 (each lane its own elements)
       const int col = wc * 32 + m32;
@@ -312,12 +347,7 @@ __global__ __launch_bounds__(256) void k_mlp_backward(int c, const int* __restri
         const int o = k0 + kq;
         p1 = mma32(sm.Z[1][(wr * 32 + m32) * LH + o], sm.W2[o * LH + wc * 32 + m32], p1);
       }
-      if (tid >= 64 && tid < 128) {
-        const int o = tid - 64;
-        float s2 = 0.f;
-        for (int r = 0; r < TA; r++) s2 += sm.Z[1][r * LH + o];
-        dvec += s2;
-      }
+      db2 += colsum16<false>(sm, sm.Z[1], wave, lane);
       __syncthreads();
       const int col = wc * 32 + m32;
 #pragma unroll
@@ -340,11 +370,7 @@ __global__ __launch_bounds__(256) void k_mlp_backward(int c, const int* __restri
         dw1b = mma32(g1[32 * (qb / 3) + m32], fb < NF ? x[fb] : 0.f, dw1b);
       }
     }
-    if (tid < 64) {
-      float s1 = 0.f;
-      for (int r = 0; r < TA; r++) s1 += sm.Z[0][r * LH + tid];
-      dvec += s1;
-    }
+    db1 += colsum16<false>(sm, sm.Z[0], wave, lane);
     __syncthreads();  // the next tile overwrites X, Z and g
   }
   float* out = partial + (size_t)blockIdx.x * NTRAIN + c * PS;
@@ -369,8 +395,13 @@ __global__ __launch_bounds__(256) void k_mlp_backward(int c, const int* __restri
       }
     }
   }
-  const int off = tid < 64 ? OB1 : (tid < 128 ? OB2 : (tid < 192 ? OB3 : OW4));
-  out[off + (tid & 63)] = dvec;
+  if (lane < 16) {
+    const int o = 16 * wave + lane;
+    out[OB1 + o] = db1;
+    out[OB2 + o] = db2;
+    out[OB3 + o] = db3;
+    out[OW4 + o] = dw4;
+  }
   if (tid == 0) out[OB4] = db4;
 }
 
@@ -422,11 +453,18 @@ __global__ __launch_bounds__(256) void k_loss_grad(const int* __restrict__ seg,
   if (threadIdx.x == 0) loss_part[s] = sl[0];
 }
 
-__global__ void k_sum(const double* __restrict__ v, int n, double* __restrict__ out) {
-  if (threadIdx.x != 0) return;
+// *out = sum of v[0..n) (one workgroup of 256, fixed order)
+__global__ __launch_bounds__(256) void k_sum(const double* __restrict__ v, int n, double* __restrict__ out) {
+  __shared__ double sh[256];
   double s = 0;
-  for (int i = 0; i < n; i++) s += v[i];
-  *out = s;
+  for (int i = threadIdx.x; i < n; i += 256) s += v[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0];
 }
 
 __global__ void k_reduce_partials(const float* __restrict__ partial, float* __restrict__ grad) {
@@ -542,7 +580,7 @@ int loss_grad_async(ltb_mlp* m) {
   ltb_ctx* c = m->ctx;
   hipLaunchKernelGGL(k_loss_grad, dim3(m->nseg), dim3(256), 0, c->stream, m->d_seg, m->d_q, m->d_t,
                      m->n, m->d_g, m->d_lpart);
-  hipLaunchKernelGGL(k_sum, dim3(1), dim3(64), 0, c->stream, m->d_lpart, m->nseg, m->d_loss);
+  hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, c->stream, m->d_lpart, m->nseg, m->d_loss);
   // both species always run: a species without atoms writes zero partials
   for (int sp = 0; sp < 2; sp++)
     hipLaunchKernelGGL(k_mlp_backward, dim3(NPB), dim3(256), 0, c->stream, sp, m->d_idx[sp], m->nidx[sp],
