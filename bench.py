@@ -10,9 +10,12 @@ value = algorithmic Chebyshev block-SpMM bytes of the timed iterations divided
 by their wall time (GB/s).  roofline = the block-SpMM kernel's algorithmic
 bytes per launch over its mean launch time (HIP events on the context stream).
 time_to_solution (default on, --no-tts): plain FP64 SCC from the neutral
-start against the learned guess with FP32 bulk and FP64 finish, both to
-|dq| < 1e-6 e, within --time-budget seconds of the start (else it says it was
-skipped); one line per SCC iteration goes to stderr meanwhile.
+start against the learned guess with FP32 bulk and FP64 finish, and against
+the learned guess with the two-level SCC (low-order iterations at M=32
+corrected by the defect of the last full-order iteration, FP32 full-order
+iterations until |dq| < 1e-2 e, FP64 after), all to |dq| < 1e-6 e on a
+full-order iteration, within --time-budget seconds of the start (else it says
+it was skipped); one line per SCC iteration goes to stderr meanwhile.
 """
 import argparse
 import json
@@ -24,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "SCC time-to-solution, 15.5k-atom SiO2: learned+mixed vs plain FP64; SpMM GB/s"
+TWO_LEVEL_M_LO = 32  # order of the low-order iterations of the two-level SCC
 HBM_PEAK_GBS = 8000.0
 T_START = time.perf_counter()
 
@@ -189,6 +193,14 @@ def time_to_solution(args):
                                  eband_fp=r["eband_fp"], mu=r["mu_ha"], fp64_iters=r["fp64_iters"],
                                  predict_s=tpred,
                                  guess_err=float(np.abs(2 * guess - runs["plain_fp64"]["q"]).max()))
+    log("learned guess + two-level SCC")
+    t0 = time.perf_counter()
+    guess = m.predict(s)
+    r = ctx.scc(args.M, guess=guess, precision=L.FP32_ACC64, switch_tol=1e-2, tol=1e-6, max_iter=2000,
+                use_graph=1, verbose=1, M_lo=TWO_LEVEL_M_LO)
+    runs["learned_mixed_two_level"] = dict(seconds=time.perf_counter() - t0, iters=r["iters"], q=r["q"],
+                                           eband_fp=r["eband_fp"], mu=r["mu_ha"], fp64_iters=r["fp64_iters"],
+                                           lo_iters=r["lo_iters"], M_lo=TWO_LEVEL_M_LO)
     if not args.no_ablation:
         log("neutral start + FP32 bulk, FP64 final iteration(s)")
         t0 = time.perf_counter()
@@ -206,6 +218,8 @@ def time_to_solution(args):
         d["dmu_vs_plain_ev"] = abs(v["mu"] - ref["mu"]) * 27.211386
         out[k] = d
     out["speedup_learned_mixed_vs_plain_fp64"] = round(ref["seconds"] / runs["learned_mixed"]["seconds"], 3)
+    out["speedup_learned_mixed_two_level_vs_plain_fp64"] = round(
+        ref["seconds"] / runs["learned_mixed_two_level"]["seconds"], 3)
     print(json.dumps(out))
     m.close()
     ctx.close()
@@ -239,8 +253,10 @@ def tts_leg(ctx, s, args, t32_iter):
                       "frac": round(k1 / HBM_PEAK_GBS, 4), "launches": st["launches"],
                       "mean_launch_us": round(st["seconds"] / max(st["launches"], 1) * 1e6, 2), "tiling": t64}
     m, info = learned.load(ctx, args.weights) if args.weights else learned.load(ctx)
-    # iteration counts measured on this workload: 73 plain, 39 learned + mixed
-    est = 1.05 * (76 * t64_iter + 42 * t32_iter + 2.0)
+    # iteration counts measured on this workload: 73 plain, 39 learned + mixed,
+    # and the two-level run's ~3 full-order + ~80 low-order ones
+    two_est = 2 * t64_iter + t32_iter + 90 * t32_iter * TWO_LEVEL_M_LO / args.M + 2.0
+    est = 1.05 * (76 * t64_iter + 42 * t32_iter + 2.0 + two_est)
     left = args.time_budget - (time.perf_counter() - T_START)
     out["estimate_s"] = round(est, 1)
     if est > left:
@@ -248,7 +264,7 @@ def tts_leg(ctx, s, args, t32_iter):
         m.close()
         return out
     log("time to solution: plain FP64 from the neutral start")
-    cap = int((left - 1.05 * (42 * t32_iter + 2.0)) / t64_iter)
+    cap = int((left - 1.05 * (42 * t32_iter + 2.0 + two_est)) / t64_iter)
     t0 = time.perf_counter()
     r = ctx.scc(args.M, precision=L.FP64, tol=1e-6, max_iter=min(400, cap), use_graph=1, check=False, verbose=1)
     plain = dict(seconds=time.perf_counter() - t0, iters=r["iters"], status=r["status"])
@@ -264,18 +280,30 @@ def tts_leg(ctx, s, args, t32_iter):
     guess = m.predict(s)
     tpred = time.perf_counter() - t0
     r = ctx.scc(args.M, guess=guess, precision=L.FP32_ACC64, switch_tol=1e-6, tol=1e-6,
-                max_iter=max(1, min(400, int(left / t32_iter) - 2)), use_graph=1, check=False, verbose=1)
+                max_iter=max(1, min(400, int((left - two_est) / t32_iter) - 2)), use_graph=1, check=False,
+                verbose=1)
     fast = dict(seconds=time.perf_counter() - t0, iters=r["iters"], fp64_iters=r["fp64_iters"],
                 status=r["status"], predict_s=round(tpred, 3),
                 guess_err_e=float(np.abs(2 * guess - qp).max()))
+    rf = r
+    log(f"time to solution: learned guess + two-level SCC (M_lo={TWO_LEVEL_M_LO}, FP32 until |dq| < 1e-2 e)")
+    t0 = time.perf_counter()
+    guess = m.predict(s)
+    tpred = time.perf_counter() - t0
+    r = ctx.scc(args.M, guess=guess, precision=L.FP32_ACC64, switch_tol=1e-2, tol=1e-6, max_iter=2000,
+                use_graph=1, check=False, verbose=1, M_lo=TWO_LEVEL_M_LO)
+    two = dict(seconds=time.perf_counter() - t0, iters=r["iters"], full_order_iters=r["iters"] - r["lo_iters"],
+               fp64_iters=r["fp64_iters"], lo_iters=r["lo_iters"], M_lo=TWO_LEVEL_M_LO, status=r["status"],
+               predict_s=round(tpred, 3))
     m.close()
     out["plain_fp64"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in plain.items()}
-    out["learned_mixed"] = {k: (round(v, 3) if isinstance(v, float) and k != "guess_err_e" else v)
-                            for k, v in fast.items()}
+    for name, d, rr in (("learned_mixed", fast, rf), ("learned_mixed_two_level", two, r)):
+        out[name] = {k: (round(v, 3) if isinstance(v, float) and k != "guess_err_e" else v) for k, v in d.items()}
+        out[name]["max_dq_vs_plain_e"] = float(np.abs(rr["q"] - qp).max())
+        out[name]["deband_vs_plain_ev_per_atom"] = abs(rr["eband_fp"] - ep) * EV / s.nat
+        out[name]["dmu_vs_plain_ev"] = abs(rr["mu_ha"] - mp) * EV
     out["speedup_learned_mixed_vs_plain_fp64"] = round(plain["seconds"] / fast["seconds"], 3)
-    out["max_dq_vs_plain_e"] = float(np.abs(r["q"] - qp).max())
-    out["deband_vs_plain_ev_per_atom"] = abs(r["eband_fp"] - ep) * EV / s.nat
-    out["dmu_vs_plain_ev"] = abs(r["mu_ha"] - mp) * EV
+    out["speedup_learned_mixed_two_level_vs_plain_fp64"] = round(plain["seconds"] / two["seconds"], 3)
     out["weights"] = info
     return out
 

@@ -198,6 +198,19 @@ typedef struct {
                          then declared on an FP64 iteration only (0: FP32 to the end) */
   int use_graph;      /* capture the device part of an iteration in a hipGraph */
   int verbose;        /* 1: one line per iteration on stderr                     */
+  /* Two-level SCC (0: off; ltb_scc_run only).  Every iteration is the same
+   * linear mixing step, but most evaluate the density cheaply: the exact-trace
+   * populations at order M_lo (< M), plus the defect F_M(q_ref) - F_M_lo(q_ref)
+   * taken at the last full-order iteration q_ref.  A full-order iteration
+   * comes first and again whenever the corrected map's own |dq| falls below
+   * lo_tol (0: tol / 10) or stops falling (its smallest value not 10% lower
+   * for 4 iterations: the FP32 rounding level, ~1.5e-7 e at 15,552 atoms),
+   * or after 400 low-order iterations; the run converges only on a full-order iteration in
+   * the final precision, so it stops at the plain solve's fixed point.  The
+   * full-order iterations run in `precision` until one has |dq| < switch_tol,
+   * then in FP64; the low-order ones always in `precision`. */
+  int M_lo;
+  double lo_tol;
 } ltb_scc_params;
 
 typedef struct {
@@ -210,6 +223,8 @@ typedef struct {
                          fixed point to first order in |q_out - q*|             */
   int graph_launches; /* iterations replayed from a captured hipGraph            */
   double seconds;     /* wall time of the loop                                   */
+  int lo_iterations;  /* two-level SCC: iterations on the corrected order-M_lo
+                         density (included in iterations)                        */
 } ltb_scc_result;
 
 /* SCC driver (main :1815-1840): loop of constructdeltaH, runKPM (bounds +

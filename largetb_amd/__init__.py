@@ -66,13 +66,14 @@ SCHED_AUTO, SCHED_BLOCKED, SCHED_SWEEP = 0, 1, 2
 class SccParams(C.Structure):
     _fields_ = [("M", C.c_int), ("kT", C.c_double), ("K", C.c_int), ("mix", C.c_double),
                 ("tol", C.c_double), ("max_iter", C.c_int), ("precision", C.c_int),
-                ("switch_tol", C.c_double), ("use_graph", C.c_int), ("verbose", C.c_int)]
+                ("switch_tol", C.c_double), ("use_graph", C.c_int), ("verbose", C.c_int),
+                ("M_lo", C.c_int), ("lo_tol", C.c_double)]
 
 
 class SccResult(C.Structure):
     _fields_ = [("iterations", C.c_int), ("fp64_iterations", C.c_int), ("dq", C.c_double),
                 ("mu_ha", C.c_double), ("eband", C.c_double), ("eband_fp", C.c_double),
-                ("graph_launches", C.c_int), ("seconds", C.c_double)]
+                ("graph_launches", C.c_int), ("seconds", C.c_double), ("lo_iterations", C.c_int)]
 
 
 class Sp2Result(C.Structure):
@@ -471,9 +472,12 @@ class Context:
         return dict(pop=pop, mu=r.mu, mu_ha=r.mu_ha, a=r.a, b=r.b, nelec=r.nelec, eband=r.eband)
 
     def scc(self, M, guess=None, kT_ev=0.05, K=8192, mix=0.2, tol=1e-6, max_iter=200,
-            precision=FP64, switch_tol=0.0, use_graph=0, check=True, verbose=0):
+            precision=FP64, switch_tol=0.0, use_graph=0, check=True, verbose=0, M_lo=0, lo_tol=0.0):
+        """ltb_scc_run.  M_lo > 0 selects the two-level SCC: most iterations
+        evaluate the density at order M_lo corrected by the defect of the last
+        full-order iteration; convergence is declared on full-order ones."""
         kT = kT_ev / HARTREE_EV if kT_ev > 0 else 0.0
-        prm = SccParams(M, kT, K, mix, tol, max_iter, precision, switch_tol, use_graph, verbose)
+        prm = SccParams(M, kT, K, mix, tol, max_iter, precision, switch_tol, use_graph, verbose, M_lo, lo_tol)
         pout = np.zeros(self.nat)
         hist = np.zeros(max_iter)
         r = SccResult()
@@ -484,7 +488,7 @@ class Context:
             self._check(st)
         return dict(status=st, pop=pout, q=2 * pout, iters=r.iterations, fp64_iters=r.fp64_iterations,
                     dq=r.dq, mu_ha=r.mu_ha, eband=r.eband, eband_fp=r.eband_fp,
-                    graph_launches=r.graph_launches, seconds=r.seconds,
+                    graph_launches=r.graph_launches, seconds=r.seconds, lo_iters=r.lo_iterations,
                     hist=hist[:r.iterations])
 
     def kpe_weights_at(self, M, mu_ha, kT_ev=0.05, K=8192):
@@ -625,7 +629,7 @@ class Batch:
     def scc(self, M, guess=None, kT_ev=0.05, K=8192, mix=0.2, tol=1e-6, max_iter=200,
             precision=FP64, switch_tol=0.0, use_graph=0, check=True, verbose=0):
         kT = kT_ev / HARTREE_EV if kT_ev > 0 else 0.0
-        prm = SccParams(M, kT, K, mix, tol, max_iter, precision, switch_tol, use_graph, verbose)
+        prm = SccParams(M, kT, K, mix, tol, max_iter, precision, switch_tol, use_graph, verbose, 0, 0.0)
         pout = np.zeros(self.nat)
         its = np.zeros(self.S, np.int32)
         ebfp = np.zeros(self.S)

@@ -859,6 +859,7 @@ int ltb_batch_scc_run(ltb_batch* b, const ltb_scc_params* prm, const double* gue
   if (prm->M < 2 || !(prm->kT > 0) || prm->K < 16 || prm->max_iter < 1 || !(prm->tol > 0) ||
       !(prm->mix > 0) || prm->mix > 1)
     return set_err(c, LTB_ERR_ARG, "bad batch SCC parameters (Fermi-Dirac kT > 0 required)");
+  if (prm->M_lo != 0) return set_err(c, LTB_ERR_ARG, "the two-level SCC (M_lo > 0) is single-structure only");
   const int M = prm->M, K = prm->K, S = b->S;
   auto t0 = std::chrono::steady_clock::now();
   int st0 = ensure_vectors(b);
@@ -979,6 +980,7 @@ int ltb_batch_scc_run(ltb_batch* b, const ltb_scc_params* prm, const double* gue
     total->eband_fp = 0;
     total->graph_launches = ngraph;
     total->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    total->lo_iterations = 0;
   }
   if (result) return set_err(c, result, "non-finite value in a batched structure");
   if (nconv < S) return set_err(c, LTB_NOT_CONVERGED, std::to_string(S - nconv) + " structures not converged");

@@ -514,6 +514,11 @@ int kpe_moments_impl(ltb_ctx* c, int M, int precision) {
   return run_moments<double>(c, M);
 }
 
+int kpe_reserve(ltb_ctx* c, int M, int precision) {
+  if (precision == LTB_FP32_ACC64) return ensure_buffers<float>(c, make_plan<float>(c), M);
+  return ensure_buffers<double>(c, make_plan<double>(c), M);
+}
+
 int kpe_time_steps(ltb_ctx* c, int precision, int steps, double* seconds, double* bytes) {
   if (precision == LTB_FP32_ACC64) return time_steps<float>(c, steps, seconds, bytes);
   return time_steps<double>(c, steps, seconds, bytes);

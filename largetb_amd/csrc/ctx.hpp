@@ -58,6 +58,12 @@ struct ltb_ctx {
   // Fermi / density
   double *d_W = nullptr, *d_X = nullptr, *d_g = nullptr, *d_c = nullptr, *d_w = nullptr;
   int fd_K = 0, fd_M = 0;
+  // the same for the low order of the two-level SCC (swapped in while its
+  // density runs, so alternating orders reallocates nothing)
+  double *lo_W = nullptr, *lo_X = nullptr, *lo_g = nullptr, *lo_w = nullptr;
+  int lo_K = 0, lo_M = 0;
+  // two-level SCC: low-order populations at q_in and the defect F_M - F_M_lo [nat]
+  double *d_plo = nullptr, *d_defect = nullptr;
   // tiling of the Chebyshev kernel per recursion precision, [0] FP32 and
   // [1] FP64 (0 / -1 fields = auto)
   ltb_kpe_tiling tiling[2] = {{0, 0, 0, 0, 0, -1, 0}, {0, 0, 0, 0, 0, -1, 0}};
@@ -101,6 +107,8 @@ void release_system(ltb_ctx* c);
 int alloc_system(ltb_ctx* c, int nat, int nnzb, int gnnz);
 int ensure_buffer(ltb_ctx* c, void** p, size_t* cap, size_t bytes);
 int kpe_moments_impl(ltb_ctx* c, int M, int precision);
+// size the recursion, partial and moment buffers for order M in this precision
+int kpe_reserve(ltb_ctx* c, int M, int precision);
 // moments r_c^T T_m r_c of explicit vectors already in d_xa (row-major [norb][ld])
 int kpe_block_moments_impl(ltb_ctx* c, int M, int nvec, int precision, const double* d_r2, double* d_momcol,
                            int ld);

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <utility>
 
 #include "ctx.hpp"
 #include "system.hpp"
@@ -314,6 +315,23 @@ __global__ void k_clear_flag(DevScalars* sc) {
   if (threadIdx.x == 0) sc->nonfinite = 0;
 }
 
+// two-level SCC: keep the low-order populations, form the defect, apply it
+__global__ void k_copy(double* __restrict__ dst, const double* __restrict__ src, int n) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a < n) dst[a] = src[a];
+}
+
+__global__ void k_defect(double* __restrict__ d, const double* __restrict__ pout,
+                         const double* __restrict__ plo, int n) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a < n) d[a] = pout[a] - plo[a];
+}
+
+__global__ void k_add_defect(double* __restrict__ pout, const double* __restrict__ d, int n) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a < n) pout[a] += d[a];
+}
+
 // gmMs kernel of getKPE (:1105-1132)
 void kernel_gmM(int M, std::vector<double>& g) {
   std::vector<double> U(M);
@@ -424,7 +442,8 @@ void free_all(ltb_ctx* c) {
   void* ptrs[] = {c->d_rowptr, c->d_col, c->d_blkrow, c->d_h0, c->d_ht64, c->d_ht32, c->d_offabs,
                   c->d_diag0, c->d_grow, c->d_gcol, c->d_gval, c->d_p0, c->d_pin, c->d_pout,
                   c->d_v, c->d_sc, c->d_red, c->d_xa, c->d_xb, c->d_part, c->d_mu1, c->d_mom,
-                  c->d_W, c->d_X, c->d_g, c->d_c, c->d_w, c->d_sa, c->d_sb, c->d_momorb, c->d_visit};
+                  c->d_W, c->d_X, c->d_g, c->d_c, c->d_w, c->d_sa, c->d_sb, c->d_momorb, c->d_visit,
+                  c->lo_W, c->lo_X, c->lo_g, c->lo_w, c->d_plo, c->d_defect};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -597,7 +616,8 @@ void release_system(ltb_ctx* c) {
   void** ps[] = {(void**)&c->d_rowptr, (void**)&c->d_col, (void**)&c->d_blkrow, (void**)&c->d_h0,
                  (void**)&c->d_ht64, (void**)&c->d_ht32, (void**)&c->d_offabs, (void**)&c->d_diag0,
                  (void**)&c->d_grow, (void**)&c->d_gcol, (void**)&c->d_gval, (void**)&c->d_p0,
-                 (void**)&c->d_pin, (void**)&c->d_pout, (void**)&c->d_v, (void**)&c->d_visit};
+                 (void**)&c->d_pin, (void**)&c->d_pout, (void**)&c->d_v, (void**)&c->d_visit,
+                 (void**)&c->d_plo, (void**)&c->d_defect};
   for (void** p : ps) {
     if (*p) (void)hipFree(*p);
     *p = nullptr;
@@ -627,6 +647,8 @@ int alloc_system(ltb_ctx* c, int nat, int nnzb, int gnnz) {
   LTB_HIP(c, hipMalloc(&c->d_pout, sizeof(double) * nat));
   LTB_HIP(c, hipMalloc(&c->d_v, sizeof(double) * nat));
   LTB_HIP(c, hipMalloc(&c->d_visit, sizeof(int) * nat));
+  LTB_HIP(c, hipMalloc(&c->d_plo, sizeof(double) * nat));
+  LTB_HIP(c, hipMalloc(&c->d_defect, sizeof(double) * nat));
   LTB_HIP(c, hipMemset(c->d_v, 0, sizeof(double) * nat));
   return LTB_OK;
 }
@@ -795,14 +817,191 @@ int scc_iteration_async(ltb_ctx* c, const ltb_scc_params* prm, const ltb_fermi_p
   return mix_async(c, prm->mix);
 }
 
+// The low-order density of the two-level SCC runs with the second set of
+// quadrature and kernel buffers swapped in (restored on every exit path).
+struct LowOrderFermi {
+  ltb_ctx* c;
+  explicit LowOrderFermi(ltb_ctx* cc) : c(cc) { swap(); }
+  ~LowOrderFermi() { swap(); }
+  void swap() {
+    std::swap(c->d_W, c->lo_W);
+    std::swap(c->d_X, c->lo_X);
+    std::swap(c->d_g, c->lo_g);
+    std::swap(c->d_w, c->lo_w);
+    std::swap(c->fd_M, c->lo_M);
+    std::swap(c->fd_K, c->lo_K);
+  }
+};
+
+// populations at order fl->M of the H~ already rescaled at q_in, into d_pout
+int low_order_density_async(ltb_ctx* c, const ltb_fermi_params* fl, int prec) {
+  LowOrderFermi sw(c);
+  int s2;
+  if ((s2 = kpe_moments_impl(c, fl->M, prec))) return s2;
+  return density_async(c, fl);
+}
+
+// two-level SCC, full-order iteration: both orders at q_in, the defect
+// between them kept, and the mixing step taken with the full-order output
+int scc_full_iteration_async(ltb_ctx* c, const ltb_scc_params* prm, const ltb_fermi_params* fp,
+                             const ltb_fermi_params* fl, int prec, int prec_lo) {
+  int s2;
+  const dim3 g(grid_for(c->nat, 256));
+  hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(64), 0, c->stream, c->d_sc);
+  if ((s2 = shift_async(c)) || (s2 = gersh_async(c)) || (s2 = rescale_async(c))) return s2;
+  if ((s2 = low_order_density_async(c, fl, prec_lo))) return s2;
+  hipLaunchKernelGGL(k_copy, g, dim3(256), 0, c->stream, c->d_plo, c->d_pout, c->nat);
+  // the full order last, so mu and the band energy on the device are its own
+  if ((s2 = kpe_moments_impl(c, prm->M, prec))) return s2;
+  if ((s2 = density_async(c, fp))) return s2;
+  hipLaunchKernelGGL(k_defect, g, dim3(256), 0, c->stream, c->d_defect, c->d_pout, c->d_plo, c->nat);
+  LTB_HIP(c, hipGetLastError());
+  return mix_async(c, prm->mix);
+}
+
+// two-level SCC, low-order iteration: F_M_lo(q_in) + defect, then mixing
+int scc_low_iteration_async(ltb_ctx* c, const ltb_scc_params* prm, const ltb_fermi_params* fl, int prec_lo) {
+  int s2;
+  hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(64), 0, c->stream, c->d_sc);
+  if ((s2 = shift_async(c)) || (s2 = gersh_async(c)) || (s2 = rescale_async(c))) return s2;
+  if ((s2 = low_order_density_async(c, fl, prec_lo))) return s2;
+  hipLaunchKernelGGL(k_add_defect, dim3(grid_for(c->nat, 256)), dim3(256), 0, c->stream, c->d_pout,
+                     c->d_defect, c->nat);
+  LTB_HIP(c, hipGetLastError());
+  return mix_async(c, prm->mix);
+}
+
+// Two-level SCC (defect correction in the expansion order).  Linear mixing
+// contracts the charge error by only (1 - mix) per iteration whatever the
+// accuracy of the density, so the density of most iterations can come from
+// order M_lo corrected by the defect d = F_M(q_ref) - F_M_lo(q_ref) of the
+// last full-order iteration at q_ref: the corrected map F_M_lo(q) + d differs
+// from F_M(q) only by (J_M - J_M_lo)(q - q_ref), second order in small
+// quantities.  Convergence is declared on full-order iterations only.
+int scc_two_level(ltb_ctx* c, const ltb_scc_params* prm, const double* guess, double* p_out, double* hist,
+                  ltb_scc_result* res) {
+  // a low-order phase ends after this many iterations even if its |dq| is
+  // still above lo_tol (linear mixing 0.2 needs ~70 from a neutral start),
+  // or once its smallest |dq| has not fallen by 10% for kStall iterations:
+  // in FP32 the corrected map's |dq| stops at the rounding level of the
+  // recursion (~1.5e-7 e at 15,552 atoms) while mixing alone shrinks it by
+  // (1 - mix) per iteration
+  constexpr int kMaxLowPhase = 400, kStall = 4;
+  auto t0 = std::chrono::steady_clock::now();
+  int st;
+  if (guess)
+    LTB_HIP(c, hipMemcpyAsync(c->d_pin, guess, sizeof(double) * c->nat, hipMemcpyHostToDevice, c->stream));
+  else
+    LTB_HIP(c, hipMemcpyAsync(c->d_pin, c->d_p0, sizeof(double) * c->nat, hipMemcpyDeviceToDevice, c->stream));
+  // buffers at their final size before anything is captured
+  if ((st = kpe_reserve(c, prm->M, prm->precision))) return st;
+  if (prm->switch_tol > 0 && (st = kpe_reserve(c, prm->M, LTB_FP64))) return st;
+  const ltb_fermi_params fp{prm->M, prm->kT, prm->K, c->nelectrons};
+  const ltb_fermi_params fl{prm->M_lo, prm->kT, prm->K, c->nelectrons};
+  const double lo_tol = prm->lo_tol > 0 ? prm->lo_tol : 0.1 * prm->tol;
+  const int prec_lo = prm->precision;
+  int prec = prm->precision;  // of the full-order iterations
+  bool full = true;
+  int it = 0, n64 = 0, nlo = 0, ngraph = 0, phase_len = 0, stall = 0;
+  double best = 0;
+  DevScalars h{};
+  const bool saved_timing = c->timing;
+  GraphExecs<1> graph;  // the low-order iteration
+  bool warmed = false;
+  st = LTB_NOT_CONVERGED;
+  for (it = 0; it < prm->max_iter; it++) {
+    int s2 = LTB_OK;
+    if (full) {
+      if ((s2 = scc_full_iteration_async(c, prm, &fp, &fl, prec, prec_lo))) return s2;
+      if (prec == LTB_FP64) n64++;
+    } else {
+      if (prm->use_graph && warmed) {
+        if (!graph.e[0]) {
+          c->timing = false;
+          hipGraph_t g = nullptr;
+          LTB_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+          s2 = scc_low_iteration_async(c, prm, &fl, prec_lo);
+          hipError_t e = hipStreamEndCapture(c->stream, &g);
+          if (s2) return s2;
+          if (e != hipSuccess) return set_err(c, LTB_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+          e = hipGraphInstantiate(&graph.e[0], g, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(g);
+          if (e != hipSuccess)
+            return set_err(c, LTB_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+        }
+        LTB_HIP(c, hipGraphLaunch(graph.e[0], c->stream));
+        ngraph++;
+      } else {
+        if ((s2 = scc_low_iteration_async(c, prm, &fl, prec_lo))) return s2;
+        warmed = true;
+      }
+      nlo++;
+      phase_len++;
+    }
+    if ((s2 = read_scalars(c, &h))) return s2;
+    if (h.nonfinite) {
+      st = set_err(c, LTB_ERR_NONFINITE, "non-finite value in SCC iteration " + std::to_string(it));
+      it++;
+      break;
+    }
+    if (hist) hist[it] = h.dq;
+    if (prm->verbose) {
+      double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      fprintf(stderr, "scc iter %d %s M=%d dq=%.3e mu=%.6f Ha eband_fp=%.10f Ha t=%.2fs\n", it,
+              (full ? prec : prec_lo) == LTB_FP64 ? "fp64" : "fp32", full ? prm->M : prm->M_lo, h.dq,
+              h.a * h.mu + h.b, h.eband_fp, el);
+      fflush(stderr);
+    }
+    if (full) {
+      const bool final_precision = prec == LTB_FP64 || prm->switch_tol <= 0;
+      if (h.dq < prm->tol && final_precision) {
+        st = LTB_OK;
+        it++;
+        break;
+      }
+      if (prec != LTB_FP64 && prm->switch_tol > 0 && h.dq < prm->switch_tol) prec = LTB_FP64;
+      full = false;
+      phase_len = 0;
+    } else {
+      if (phase_len == 1 || h.dq < 0.9 * best) {
+        best = h.dq;
+        stall = 0;
+      } else {
+        stall++;
+      }
+      if (h.dq < lo_tol || stall >= kStall || phase_len >= kMaxLowPhase) full = true;
+    }
+  }
+  c->timing = saved_timing;
+  // d_pout holds the last iteration's output: F_M(q_in) after a full-order one
+  if (p_out) LTB_HIP(c, hipMemcpy(p_out, c->d_pout, sizeof(double) * c->nat, hipMemcpyDeviceToHost));
+  auto t1 = std::chrono::steady_clock::now();
+  if (res) {
+    res->iterations = it;
+    res->fp64_iterations = n64;
+    res->lo_iterations = nlo;
+    res->dq = h.dq;
+    res->mu_ha = h.a * h.mu + h.b;
+    res->eband = h.eband;
+    res->eband_fp = h.eband_fp;
+    res->graph_launches = ngraph;
+    res->seconds = std::chrono::duration<double>(t1 - t0).count();
+  }
+  if (st == LTB_NOT_CONVERGED)
+    set_err(c, LTB_NOT_CONVERGED, "SCC not converged after " + std::to_string(prm->max_iter) + " iterations");
+  return st;
+}
+
 int ltb_scc_run(ltb_ctx* c, const ltb_scc_params* prm, const double* guess, double* p_out,
                 double* hist, ltb_scc_result* res) {
   int st = need_system(c);
   if (st) return st;
   if (!prm || prm->M < 2 || prm->max_iter < 1 || !(prm->tol > 0) || !(prm->mix > 0) || prm->mix > 1 ||
       (prm->kT > 0 && prm->K < 16) ||
-      (prm->precision != LTB_FP64 && prm->precision != LTB_FP32_ACC64))
+      (prm->precision != LTB_FP64 && prm->precision != LTB_FP32_ACC64) ||
+      prm->M_lo < 0 || (prm->M_lo > 0 && (prm->M_lo < 2 || prm->M_lo >= prm->M)) || !(prm->lo_tol >= 0))
     return set_err(c, LTB_ERR_ARG, "bad SCC parameters");
+  if (prm->M_lo > 0) return scc_two_level(c, prm, guess, p_out, hist, res);
   auto t0 = std::chrono::steady_clock::now();
   if (guess)
     LTB_HIP(c, hipMemcpyAsync(c->d_pin, guess, sizeof(double) * c->nat, hipMemcpyHostToDevice, c->stream));
@@ -874,6 +1073,7 @@ int ltb_scc_run(ltb_ctx* c, const ltb_scc_params* prm, const double* guess, doub
     res->eband_fp = h.eband_fp;
     res->graph_launches = ngraph;
     res->seconds = std::chrono::duration<double>(t1 - t0).count();
+    res->lo_iterations = 0;
   }
   if (st == LTB_NOT_CONVERGED)
     set_err(c, LTB_NOT_CONVERGED, "SCC not converged after " + std::to_string(prm->max_iter) + " iterations");

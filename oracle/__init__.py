@@ -12,6 +12,11 @@ Pins of the numpy functions (the C functions' pins are in oracle.c):
   (tests/test_stochastic_host.py).
 * gaussian_stream: reproduces the atom displacements that orc_quartz draws
   (displacement / sigma, 1e-12).
+* System.scc_two_level (the two-level SCC of ltb_scc_run with M_lo > 0, a
+  loop over orc_density): converges to the fixed point of the plain orc_scc
+  at the full order, from the neutral start and from a perturbed one, and a
+  planted error in its defect (dropped, or taken from the wrong order) moves
+  the answer or stalls it (tests/test_oracle_math.py).
 """
 import ctypes as C
 import os
@@ -266,6 +271,59 @@ class System:
         return dict(status=st, pop=pio, q=2 * pio, iters=its.value, hist=hist[:its.value],
                     mu=res.mu, mu_ha=res.mu_ha, a=res.a, b=res.b, eband=res.eband,
                     nelec=res.nelec)
+
+    def scc_two_level(self, M, M_lo, **kw):
+        return _scc_two_level(self, M, M_lo, **kw)
+
+
+def _scc_two_level(o, M, M_lo, guess=None, kT_ev=0.05, K=8192, mix=0.2, tol=1e-6, lo_tol=0.0, maxit=2000,
+                   nthreads=0, defect_order=None, max_phase=400):
+    """Two-level SCC in FP64 (the control flow of ltb_scc_run with M_lo > 0):
+    a full-order iteration evaluates F_M and F_M_lo at q_in, keeps the defect
+    d = F_M - F_M_lo and mixes with F_M; low-order iterations mix with
+    F_M_lo + d until their |dq| < lo_tol, or their smallest |dq| has not
+    fallen by 10% for 4 iterations, or max_phase of them; convergence only on
+    a full-order iteration with |dq| < tol.  defect_order (test hook) takes
+    the defect's low-order term at another order."""
+    lo_tol = lo_tol if lo_tol > 0 else 0.1 * tol
+    G = o.dense_gamma()
+    p0 = o.p0
+    pin = np.array(p0 if guess is None else guess, np.float64)
+    d = np.zeros(o.nat)
+    full, phase, n_lo, hist, full_hist = True, 0, 0, [], []
+    best, stall = 0.0, 0
+    status = 1
+    for it in range(maxit):
+        shift = np.repeat(G @ (pin - p0), 4)
+        if full:
+            lo = o.density(defect_order or M_lo, shift=shift, kT_ev=kT_ev, K=K, nthreads=nthreads)
+            r = o.density(M, shift=shift, kT_ev=kT_ev, K=K, nthreads=nthreads)
+            pout = r["pop"]
+            d = pout - lo["pop"]
+        else:
+            r = o.density(M_lo, shift=shift, kT_ev=kT_ev, K=K, nthreads=nthreads)
+            pout = r["pop"] + d
+            n_lo += 1
+            phase += 1
+        dq = 2 * np.abs(pout - pin).max()
+        hist.append(dq)
+        if full:
+            full_hist.append(dq)
+        if full and dq < tol:
+            status = 0
+            break
+        pin = pin + mix * (pout - pin)
+        if full:
+            full, phase = False, 0
+        else:
+            if phase == 1 or dq < 0.9 * best:
+                best, stall = dq, 0
+            else:
+                stall += 1
+            if dq < lo_tol or stall >= 4 or phase >= max_phase:
+                full = True
+    return dict(status=status, pop=pout, q=2 * pout, iters=len(hist), lo_iters=n_lo, hist=np.array(hist),
+                full_hist=np.array(full_hist), mu_ha=r["mu_ha"], eband=r["eband"])
 
 
 def gmM(M):

@@ -179,6 +179,42 @@ def test_scc_fixed_point_and_guess_independence(c0):
     assert r3["status"] == 5  # ORC_NOT_CONVERGED
 
 
+def test_two_level_scc_reaches_the_plain_fixed_point(c0):
+    """The two-level SCC (low-order iterations corrected by the defect of the
+    last full-order one) stops at the plain SCC's fixed point: the final
+    full-order iteration satisfies the same |dq| < 1e-6 e test, so the
+    charges agree to well inside it, from the neutral start and from a
+    perturbed guess, with most iterations at the low order."""
+    plain = c0.scc(200, tol=1e-6, maxit=300)
+    ref = c0.scc(200, tol=1e-10, maxit=600)  # the fixed point itself
+    assert plain["status"] == 0 and ref["status"] == 0
+    rng = np.random.default_rng(5)
+    noise = rng.standard_normal(c0.nat)
+    noise -= noise.mean()
+    guess = ref["pop"] + 1.5e-3 * noise / np.abs(noise).max()  # 3e-3 e off
+    for g in (None, guess):
+        r = c0.scc_two_level(200, 32, guess=g, tol=1e-6)
+        full = r["iters"] - r["lo_iters"]
+        assert r["status"] == 0
+        assert np.abs(r["q"] - ref["q"]).max() < 1e-7
+        assert np.abs(r["q"] - plain["q"]).max() < 1e-6
+        assert r["hist"][-1] < 1e-6 and len(r["full_hist"]) == full
+        assert 2 <= full <= 5 and r["lo_iters"] > 5 * full
+    assert abs(r["eband"] - ref["eband"]) * HA / c0.nat < 1e-6
+
+
+@pytest.mark.parametrize("planted", ["no defect", "defect from another order"])
+def test_two_level_scc_planted_defect_errors_do_not_converge(c0, planted):
+    """The defect is what moves the low-order fixed point onto the full-order
+    one: without it (its low-order term taken at the full order, so it is
+    zero) or taken at another order than the low-order map's, every
+    full-order check fails by about the order error and the run stalls."""
+    order = 200 if planted == "no defect" else 64
+    r = c0.scc_two_level(200, 32, tol=1e-6, maxit=250, defect_order=order, max_phase=60)
+    assert r["status"] != 0
+    assert len(r["full_hist"]) >= 3 and r["full_hist"].min() > 1e-5
+
+
 def test_sp2_dense_is_the_projector(c0):
     H = c0.dense_h()
     lo, hi = c0.gershgorin()
