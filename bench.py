@@ -208,6 +208,13 @@ def time_to_solution(args):
                     use_graph=1, verbose=1)
         runs["neutral_mixed"] = dict(seconds=time.perf_counter() - t0, iters=r["iters"], q=r["q"],
                                      eband_fp=r["eband_fp"], mu=r["mu_ha"])
+        log("neutral start + two-level SCC")
+        t0 = time.perf_counter()
+        r = ctx.scc(args.M, precision=L.FP32_ACC64, switch_tol=1e-2, tol=1e-6, max_iter=2000,
+                    use_graph=1, verbose=1, M_lo=TWO_LEVEL_M_LO)
+        runs["neutral_two_level"] = dict(seconds=time.perf_counter() - t0, iters=r["iters"], q=r["q"],
+                                         eband_fp=r["eband_fp"], mu=r["mu_ha"], fp64_iters=r["fp64_iters"],
+                                         lo_iters=r["lo_iters"], M_lo=TWO_LEVEL_M_LO)
     ref = runs["plain_fp64"]
     out = {"metric": METRIC, "workload": f"{s.nat}-atom alpha-quartz {'x'.join(map(str, args.cells))}",
            "M": args.M, "sigma_ang": args.sigma, "seed": args.seed, "tiling_fp32": t32, "tiling_fp64": t64}
@@ -222,6 +229,52 @@ def time_to_solution(args):
         ref["seconds"] / runs["learned_mixed_two_level"]["seconds"], 3)
     print(json.dumps(out))
     m.close()
+    ctx.close()
+
+
+def two_level_check(args):
+    """The learned guess + two-level SCC on one structure (both tilings tuned,
+    both code paths warmed up first), then one plain FP64 iteration started
+    from its answer: that iteration's |dq| is the distance of the answer from
+    the plain solve's fixed point (the plain solve itself stops within 1e-6 e
+    of it).  For cells whose plain solve is too long to repeat."""
+    import numpy as np
+    import largetb_amd as L
+    from largetb_amd import learned
+    EV = 27.211386
+    s = L.System.quartz(*args.cells, sigma_ang=args.sigma, seed=args.seed)
+    ctx = L.Context(0)
+    ctx.load(s)
+    ctx.set_populations(None)
+    lo, hi = ctx.gershgorin()
+    ctx.rescale((hi - lo) / 2, (hi + lo) / 2)
+    t64, _ = ctx.autotune(L.FP64, 3)
+    t32, _ = ctx.autotune(L.FP32_ACC64, 3)
+    for p in (L.FP32_ACC64, L.FP64):
+        ctx.scc(args.M, precision=p, max_iter=1, tol=1e-30, check=False)
+    ctx.scc(args.M, precision=L.FP32_ACC64, max_iter=2, tol=1e-30, check=False, M_lo=TWO_LEVEL_M_LO)
+    m, info = learned.load(ctx, args.weights) if args.weights else learned.load(ctx)
+    t0 = time.perf_counter()
+    guess = m.predict(s)
+    tpred = time.perf_counter() - t0
+    r = ctx.scc(args.M, guess=guess, precision=L.FP32_ACC64, switch_tol=1e-2, tol=1e-6, max_iter=2000,
+                use_graph=1, verbose=1, M_lo=TWO_LEVEL_M_LO)
+    secs = time.perf_counter() - t0
+    m.close()
+    # one plain FP64 iteration from the answer (per-spin populations q_out / 2)
+    chk = ctx.scc(args.M, guess=r["pop"], precision=L.FP64, tol=1e-30, max_iter=1, check=False)
+    out = {"workload": f"{s.nat}-atom alpha-quartz {'x'.join(map(str, args.cells))}, M={args.M}, "
+                       f"sigma {args.sigma} A, seed {args.seed}, linear mixing 0.2, |dq| < 1e-6 e",
+           "run": f"learned guess + two-level SCC (M_lo={TWO_LEVEL_M_LO}, FP32 until |dq| < 1e-2 e, FP64 after)",
+           "seconds": round(secs, 3), "predict_s": round(tpred, 3), "status": r["status"], "iters": r["iters"],
+           "full_order_iters": r["iters"] - r["lo_iters"], "fp64_iters": r["fp64_iters"],
+           "lo_iters": r["lo_iters"], "guess_err_vs_answer_e": float(np.abs(2 * guess - r["q"]).max()),
+           "plain_fp64_check": {"dq_e": chk["hist"][0], "seconds": round(chk["seconds"], 3),
+                                "deband_ev_per_atom": abs(chk["eband_fp"] - r["eband_fp"]) * EV / s.nat,
+                                "dmu_ev": abs(chk["mu_ha"] - r["mu_ha"]) * EV},
+           "tiling_fp32": t32, "tiling_fp64": t64, "weights": info,
+           "command": "python3 -u bench.py " + " ".join(sys.argv[1:])}
+    print(json.dumps(out))
     ctx.close()
 
 
@@ -488,12 +541,19 @@ def batch_measure(args):
     else:
         graph = b.scc(M, use_graph=1, verbose=1, **kw)
         log(f"graph: {graph['batch_iters']} batch iterations, {graph['seconds']:.2f} s")
+    # the same batch with the two-level SCC (FP32 structures: low-order
+    # iterations at M_lo between full-order ones, plain FP64 finish)
+    b.scc(M, max_iter=2, tol=1e-30, check=False, M_lo=TWO_LEVEL_M_LO)  # warm-up of its code path
+    two = b.scc(M, use_graph=1, verbose=1, M_lo=TWO_LEVEL_M_LO, **dict(kw, max_iter=max(kw["max_iter"], 2000)))
+    log(f"two-level: {two['batch_iters']} batch iterations ({two['lo_iters']} low-order only), "
+        f"{two['seconds']:.2f} s")
     qb = b.split(graph["q"])
+    qt = b.split(two["q"])
     b.close()
     # the same cells one at a time (single-structure SCC, same settings, graphs)
     nseq = len(systems) if args.batch_seq <= 0 else min(args.batch_seq, len(systems))
     pick = np.unique(np.linspace(0, len(systems) - 1, nseq).round().astype(int))
-    tseq, err, eerr = 0.0, 0.0, 0.0
+    tseq, err, eerr, err2, eerr2 = 0.0, 0.0, 0.0, 0.0, 0.0
     for k in pick:
         ctx.load(systems[k])
         t1 = time.perf_counter()
@@ -502,6 +562,8 @@ def batch_measure(args):
         ref = ctx.scc(M, precision=L.FP64, tol=1e-6, max_iter=args.max_iter, use_graph=1, check=False)
         err = max(err, float(np.abs(qb[k] - ref["q"]).max()))
         eerr = max(eerr, abs(graph["eband_fp"][k] - ref["eband_fp"]) * 27.211386 / systems[k].nat)
+        err2 = max(err2, float(np.abs(qt[k] - ref["q"]).max()))
+        eerr2 = max(eerr2, abs(two["eband_fp"][k] - ref["eband_fp"]) * 27.211386 / systems[k].nat)
     ctx.close()
     seq_all = tseq / len(pick) * len(systems)
     achieved = st["bytes"] / st["seconds"] / 1e9 if st["seconds"] > 0 else None
@@ -518,6 +580,13 @@ def batch_measure(args):
            "one_at_a_time_sample": f"{len(pick)} of {len(systems)} cells, scaled to all",
            "speedup_vs_one_at_a_time": round(seq_all / graph["seconds"], 2),
            "max_dq_vs_fp64_e": err, "max_deband_vs_fp64_ev_per_atom": eerr,
+           "two_level": {"M_lo": TWO_LEVEL_M_LO, "status": two["status"], "batch_iterations": two["batch_iters"],
+                         "low_order_only_batch_iterations": two["lo_iters"],
+                         "fp64_batch_iterations": two["fp64_iters"], "seconds_graph": round(two["seconds"], 3),
+                         "cells_per_s": round(len(systems) / two["seconds"], 2),
+                         "speedup_vs_batch": round(graph["seconds"] / two["seconds"], 2),
+                         "speedup_vs_one_at_a_time": round(seq_all / two["seconds"], 2),
+                         "max_dq_vs_fp64_e": err2, "max_deband_vs_fp64_ev_per_atom": eerr2},
            "roofline": {"bound": "hbm", "kernel": "k_batch_step", "unit": "GB/s", "peak": HBM_PEAK_GBS,
                         "achieved": round(achieved, 1) if achieved else None,
                         "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
@@ -647,6 +716,8 @@ def main():
                     help="8x8x8 report: FP32 forms vs FP64 (KPE density, SCC, SP2) with K1 and MFMA rates")
     ap.add_argument("--tts", action="store_true",
                     help="SCC time-to-solution: plain FP64 neutral start vs learned guess + mixed")
+    ap.add_argument("--two-level", action="store_true",
+                    help="learned guess + two-level SCC, checked by one plain FP64 iteration from its answer")
     ap.add_argument("--batch", type=int, default=0,
                     help="batched SCC over this many uneven cells (M=256 unless --M is given)")
     ap.add_argument("--no-graph", action="store_true",
@@ -689,6 +760,9 @@ def main():
         return
     if args.tts:
         time_to_solution(args)
+        return
+    if args.two_level:
+        two_level_check(args)
         return
     if args.sp2:
         sp2_measure(args)

@@ -261,7 +261,15 @@ int ltb_batch_sizes(const ltb_batch* b, int* nstruct, int* natoms_total, int* at
  * [nstruct] (each optional): iterations, eband_fp (Hartree, as ltb_scc_result),
  * mu_ha and status (LTB_OK, LTB_NOT_CONVERGED, LTB_ERR_NONFINITE).  total:
  * iterations of the whole batch, those with an FP64 launch, graph replays,
- * wall time.  Returns LTB_NOT_CONVERGED if any structure hit max_iter. */
+ * wall time.  Returns LTB_NOT_CONVERGED if any structure hit max_iter.
+ * Two-level (prm->M_lo > 0): a structure recursing in FP32 runs the two-level
+ * SCC of ltb_scc_run (low-order iterations at M_lo corrected by the defect of
+ * its last full-order iteration, whose order-M_lo populations come from the
+ * first M_lo + 1 moments of the same recursion) until a full-order iteration
+ * has |dq| < switch_tol, then plain FP64 iterations; the recursion of a
+ * batch iteration stops after step M_lo/2 for the structures in a low-order
+ * iteration.  total->lo_iterations counts batch iterations with only
+ * low-order (or no) recursions past step M_lo/2. */
 int ltb_batch_scc_run(ltb_batch* b, const ltb_scc_params* prm, const double* p_guess, double* p_out,
                       int* iterations, double* eband_fp, double* mu_ha, int* status,
                       ltb_scc_result* total);

@@ -627,9 +627,12 @@ class Batch:
         self.close()
 
     def scc(self, M, guess=None, kT_ev=0.05, K=8192, mix=0.2, tol=1e-6, max_iter=200,
-            precision=FP64, switch_tol=0.0, use_graph=0, check=True, verbose=0):
+            precision=FP64, switch_tol=0.0, use_graph=0, check=True, verbose=0, M_lo=0, lo_tol=0.0):
+        """ltb_batch_scc_run.  M_lo > 0: structures recursing in FP32 run the
+        two-level SCC until a full-order iteration has |dq| < switch_tol, then
+        plain FP64 iterations."""
         kT = kT_ev / HARTREE_EV if kT_ev > 0 else 0.0
-        prm = SccParams(M, kT, K, mix, tol, max_iter, precision, switch_tol, use_graph, verbose, 0, 0.0)
+        prm = SccParams(M, kT, K, mix, tol, max_iter, precision, switch_tol, use_graph, verbose, M_lo, lo_tol)
         pout = np.zeros(self.nat)
         its = np.zeros(self.S, np.int32)
         ebfp = np.zeros(self.S)
@@ -643,7 +646,7 @@ class Batch:
             self.ctx._check(st)
         return dict(status=st, pop=pout, q=2 * pout, iters=its, eband_fp=ebfp, mu_ha=mu,
                     struct_status=status, batch_iters=r.iterations, fp64_iters=r.fp64_iterations,
-                    graph_launches=r.graph_launches, seconds=r.seconds)
+                    graph_launches=r.graph_launches, seconds=r.seconds, lo_iters=r.lo_iterations)
 
     def tiling(self):
         t = KpeTiling()

@@ -448,6 +448,27 @@ __global__ void k_batch_shift(const int* __restrict__ grow, const int* __restric
   v[i] = acc;
 }
 
+// two-level SCC, per atom by its structure's mode (1 low-order iteration,
+// 2 full-order iteration): after the order-M_lo populations, a low-order
+// iteration adds the defect and a full-order one keeps them
+__global__ void k_batch_lo_combine(double* __restrict__ pout, double* __restrict__ plo,
+                                   const double* __restrict__ defect, const int* __restrict__ atomst,
+                                   const int* __restrict__ mode, int nat) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= nat) return;
+  const int md = mode[atomst[a]];
+  if (md == 1) pout[a] += defect[a];
+  if (md == 2) plo[a] = pout[a];
+}
+
+// after the order-M populations: the defect of a full-order iteration
+__global__ void k_batch_defect(const double* __restrict__ pout, const double* __restrict__ plo,
+                               double* __restrict__ defect, const int* __restrict__ atomst,
+                               const int* __restrict__ mode, int nat) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a < nat && mode[atomst[a]] == 2) defect[a] = pout[a] - plo[a];
+}
+
 void kernel_gmM(int M, std::vector<double>& g) {
   std::vector<double> U(M);
   const double alpha = kPi / (M + 2), lam = std::cos(alpha);
@@ -488,6 +509,14 @@ struct ltb_batch {
   double *d_part = nullptr, *d_mu1 = nullptr, *d_mom = nullptr, *d_tot = nullptr, *d_W = nullptr,
          *d_X = nullptr, *d_g = nullptr, *d_w = nullptr;
   int M = -1, K = -1;
+  // two-level SCC: the order-M_lo quadrature state, per-structure masks
+  // (act_full: recursing past step M_lo/2 this iteration; act_low: finishing
+  // at order M_lo; mode: 1 low-order, 2 full-order two-level iteration) and
+  // per-atom low-order populations and defect
+  double *d_tot_lo = nullptr, *d_g_lo = nullptr, *d_w_lo = nullptr;
+  int M_lo = -1;
+  int* d_masks = nullptr;  // [3][S]: act_full, act_low, mode
+  double *d_plo = nullptr, *d_defect = nullptr;
   std::vector<double> Ne;
   std::vector<int> nnzb_s;
   // step-kernel statistics of the last ltb_batch_scc_run (events only in eager
@@ -520,15 +549,14 @@ int bupload(ltb_batch* b, P** p, const std::vector<P>& h) {
 // step, the vectors read by the gathers (t_n), read (t_{n-1}) and written
 // (t_{n+1}) once over the valid columns (1, 2, 3 streams for steps 0, 1, >1),
 // plus H~ blocks, block-column indices and row pointers once
-double iteration_bytes(const ltb_batch* b, const std::vector<int>& active, int M) {
-  const int nsteps = (M + 1) / 2;
+double iteration_bytes(const ltb_batch* b, const std::vector<int>& active, int n0, int n1) {
   double bytes = 0;
   for (int s = 0; s < b->S; s++) {
     if (!active[s]) continue;
     const double el = active[s] == 1 ? 4.0 : 8.0;
     const double norb = 4.0 * (b->aoff[s + 1] - b->aoff[s]);
     const double hb = b->nnzb_s[s] * (16.0 * el + 4.0) + 4.0 * (norb / 4 + 1);
-    for (int n = 0; n < nsteps; n++) {
+    for (int n = n0; n < n1; n++) {
       const int streams = n == 0 ? 1 : (n == 1 ? 2 : 3);
       bytes += streams * el * norb * norb + hb;
     }
@@ -667,9 +695,10 @@ void launch_step(ltb_batch* b, int mode, int grid, const void* cur, void* nxt) {
     launch_step<T, 2>(b, grid, cur, nxt);
 }
 
-// the exact-trace moments of every active structure; `precs` bit 0: some
-// structure recurses in FP32, bit 1: some in FP64
-int batch_moments(ltb_batch* b, int M, int precs, bool capturing) {
+// recursion steps n0 <= n < n1 of the exact-trace moments (order M) of the
+// structures active in `act` (the tasks are marked from it first); `precs`
+// bit 0: some structure recurses in FP32, bit 1: some in FP64
+int batch_steps(ltb_batch* b, int M, int n0, int n1, int precs, const int* act, bool capturing) {
   ltb_ctx* c = b->ctx;
   const int nsteps = (M + 1) / 2;
   const int grid = step_grid(b);
@@ -681,7 +710,9 @@ int batch_moments(ltb_batch* b, int M, int precs, bool capturing) {
       b->events.push_back(e);
     }
   }
-  for (int n = 0; n < nsteps; n++) {
+  hipLaunchKernelGGL(k_batch_mark, dim3((b->nslots + 255) / 256), dim3(256), 0, c->stream,
+                     (BatchTask*)b->d_tasks, b->nslots, act);
+  for (int n = n0; n < n1; n++) {
     const int mode = n == 0 ? 0 : (n == 1 ? 1 : 2);
     void *cur, *nxt;
     if (timed) LTB_HIP(c, hipEventRecord(b->events[2 * n], c->stream));
@@ -695,18 +726,41 @@ int batch_moments(ltb_batch* b, int M, int precs, bool capturing) {
     }
     if (timed) LTB_HIP(c, hipEventRecord(b->events[2 * n + 1], c->stream));
     hipLaunchKernelGGL(k_batch_reduce, dim3((b->ncols + 255) / 256), dim3(256), 0, c->stream, b->d_part,
-                       b->ncols, b->d_gc_s, b->d_gc_c, b->d_tbegin, b->d_nsg, b->d_aoff, b->d_active, n,
+                       b->ncols, b->d_gc_s, b->d_gc_c, b->d_tbegin, b->d_nsg, b->d_aoff, act, n,
                        M, b->d_mu1, b->d_mom, b->nat);
   }
-  hipLaunchKernelGGL(k_batch_momtot, dim3(M + 1, b->S), dim3(256), 0, c->stream, b->d_mom, b->nat,
-                     b->d_aoff, b->d_active, M, b->d_tot);
   LTB_HIP(c, hipGetLastError());
   return LTB_OK;
 }
 
-int batch_iteration(ltb_batch* b, const ltb_scc_params* prm, int precs, bool capturing) {
+// from the moment totals to the populations at order M of the structures
+// active in `act`, then their |dq|, band energy and mixing
+void batch_density(ltb_batch* b, const ltb_scc_params* prm, int M, const double* tot, const double* g, double* w,
+                   const int* act) {
   ltb_ctx* c = b->ctx;
-  const int M = prm->M, K = prm->K;
+  const int K = prm->K;
+  hipLaunchKernelGGL(k_batch_weights, dim3((K + 255) / 256, b->S), dim3(256), 0, c->stream, tot, g, M, K, act,
+                     b->d_W, b->d_X);
+  hipLaunchKernelGGL(k_batch_bisect, dim3(b->S), dim3(1024), 0, c->stream, b->d_W, b->d_X, K, prm->kT, act,
+                     b->d_sc);
+  hipLaunchKernelGGL(k_batch_coeffs, dim3(M, b->S), dim3(256), 0, c->stream, b->d_X, K, g, M, prm->kT, act,
+                     b->d_sc, w);
+  hipLaunchKernelGGL(k_batch_pops, dim3((b->nat + 255) / 256), dim3(256), 0, c->stream, b->d_mom, w, M, b->nat,
+                     b->d_atomst, act, b->d_pout);
+}
+
+void batch_finish(ltb_batch* b, const ltb_scc_params* prm, int M, const double* tot, const double* w,
+                  const int* act) {
+  hipLaunchKernelGGL(k_batch_finish, dim3(b->S), dim3(256), 0, b->ctx->stream, tot, w, M, b->d_aoff, b->d_grow,
+                     b->d_gcol, b->d_gval, b->d_pin, b->d_pout, prm->mix, act, b->d_sc);
+}
+
+// One SCC iteration of the batch.  precs / precs_full: precisions of the
+// structures recursing up to step M_lo/2 / past it (two-level only; without
+// it every structure recurses to M and precs_full is unused).
+int batch_iteration(ltb_batch* b, const ltb_scc_params* prm, int precs, int precs_full, bool capturing) {
+  ltb_ctx* c = b->ctx;
+  const int M = prm->M, nsteps = (M + 1) / 2;
   hipLaunchKernelGGL(k_batch_shift, dim3((b->nat + 255) / 256), dim3(256), 0, c->stream, b->d_grow,
                      b->d_gcol, b->d_gval, b->d_pin, b->d_p0, b->d_atomst, b->d_active, b->d_v, b->nat);
   hipLaunchKernelGGL(k_batch_gersh, dim3(b->S), dim3(256), 0, c->stream, b->d_diag0, b->d_offabs, b->d_v,
@@ -714,20 +768,38 @@ int batch_iteration(ltb_batch* b, const ltb_scc_params* prm, int precs, bool cap
   hipLaunchKernelGGL(k_batch_rescale, dim3((b->nnzb * 16 + 255) / 256), dim3(256), 0, c->stream, b->d_h0,
                      b->d_blkrow, b->d_col, b->d_atomst, b->d_v, b->d_sc, b->d_active, b->nnzb, b->d_ht64,
                      b->d_ht32);
-  hipLaunchKernelGGL(k_batch_mark, dim3((b->nslots + 255) / 256), dim3(256), 0, c->stream,
-                     (BatchTask*)b->d_tasks, b->nslots, b->d_active);
-  int st = batch_moments(b, M, precs, capturing);
-  if (st) return st;
-  hipLaunchKernelGGL(k_batch_weights, dim3((K + 255) / 256, b->S), dim3(256), 0, c->stream, b->d_tot,
-                     b->d_g, M, K, b->d_active, b->d_W, b->d_X);
-  hipLaunchKernelGGL(k_batch_bisect, dim3(b->S), dim3(1024), 0, c->stream, b->d_W, b->d_X, K, prm->kT,
-                     b->d_active, b->d_sc);
-  hipLaunchKernelGGL(k_batch_coeffs, dim3(M, b->S), dim3(256), 0, c->stream, b->d_X, K, b->d_g, M,
-                     prm->kT, b->d_active, b->d_sc, b->d_w);
-  hipLaunchKernelGGL(k_batch_pops, dim3((b->nat + 255) / 256), dim3(256), 0, c->stream, b->d_mom, b->d_w,
-                     M, b->nat, b->d_atomst, b->d_active, b->d_pout);
-  hipLaunchKernelGGL(k_batch_finish, dim3(b->S), dim3(256), 0, c->stream, b->d_tot, b->d_w, M, b->d_aoff,
-                     b->d_grow, b->d_gcol, b->d_gval, b->d_pin, b->d_pout, prm->mix, b->d_active, b->d_sc);
+  int st;
+  if (prm->M_lo <= 0) {
+    if ((st = batch_steps(b, M, 0, nsteps, precs, b->d_active, capturing))) return st;
+    hipLaunchKernelGGL(k_batch_momtot, dim3(M + 1, b->S), dim3(256), 0, c->stream, b->d_mom, b->nat,
+                       b->d_aoff, b->d_active, M, b->d_tot);
+    batch_density(b, prm, M, b->d_tot, b->d_g, b->d_w, b->d_active);
+    batch_finish(b, prm, M, b->d_tot, b->d_w, b->d_active);
+    LTB_HIP(c, hipGetLastError());
+    return LTB_OK;
+  }
+  // two-level: every structure to step M_lo/2, those in a full-order (or
+  // plain) iteration on to M/2; order-M_lo populations for all (a low-order
+  // iteration adds the defect and finishes there), order M for the others
+  const int M_lo = prm->M_lo, nlo = (M_lo + 1) / 2;
+  const int *act_full = b->d_masks, *act_low = b->d_masks + b->S, *mode = b->d_masks + 2 * b->S;
+  const dim3 ga((b->nat + 255) / 256);
+  if ((st = batch_steps(b, M, 0, nlo, precs, b->d_active, capturing))) return st;
+  if (precs_full && (st = batch_steps(b, M, nlo, nsteps, precs_full, act_full, capturing))) return st;
+  hipLaunchKernelGGL(k_batch_momtot, dim3(M_lo + 1, b->S), dim3(256), 0, c->stream, b->d_mom, b->nat,
+                     b->d_aoff, b->d_active, M_lo, b->d_tot_lo);
+  batch_density(b, prm, M_lo, b->d_tot_lo, b->d_g_lo, b->d_w_lo, b->d_active);
+  hipLaunchKernelGGL(k_batch_lo_combine, ga, dim3(256), 0, c->stream, b->d_pout, b->d_plo, b->d_defect,
+                     b->d_atomst, mode, b->nat);
+  batch_finish(b, prm, M_lo, b->d_tot_lo, b->d_w_lo, act_low);
+  if (precs_full) {
+    hipLaunchKernelGGL(k_batch_momtot, dim3(M + 1, b->S), dim3(256), 0, c->stream, b->d_mom, b->nat,
+                       b->d_aoff, act_full, M, b->d_tot);
+    batch_density(b, prm, M, b->d_tot, b->d_g, b->d_w, act_full);
+    hipLaunchKernelGGL(k_batch_defect, ga, dim3(256), 0, c->stream, b->d_pout, b->d_plo, b->d_defect,
+                       b->d_atomst, mode, b->nat);
+    batch_finish(b, prm, M, b->d_tot, b->d_w, act_full);
+  }
   LTB_HIP(c, hipGetLastError());
   return LTB_OK;
 }
@@ -847,7 +919,8 @@ void ltb_batch_destroy(ltb_batch* b) {
   free_tasks(b);
   for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
   for (void* p : {b->d_x32[0], b->d_x32[1], b->d_x64[0], b->d_x64[1], (void*)b->d_mom, (void*)b->d_tot, (void*)b->d_W,
-                  (void*)b->d_X, (void*)b->d_g, (void*)b->d_w})
+                  (void*)b->d_X, (void*)b->d_g, (void*)b->d_w, (void*)b->d_tot_lo, (void*)b->d_g_lo,
+                  (void*)b->d_w_lo})
     if (p) (void)hipFree(p);
   delete b;
 }
@@ -859,8 +932,10 @@ int ltb_batch_scc_run(ltb_batch* b, const ltb_scc_params* prm, const double* gue
   if (prm->M < 2 || !(prm->kT > 0) || prm->K < 16 || prm->max_iter < 1 || !(prm->tol > 0) ||
       !(prm->mix > 0) || prm->mix > 1)
     return set_err(c, LTB_ERR_ARG, "bad batch SCC parameters (Fermi-Dirac kT > 0 required)");
-  if (prm->M_lo != 0) return set_err(c, LTB_ERR_ARG, "the two-level SCC (M_lo > 0) is single-structure only");
+  if (prm->M_lo < 0 || (prm->M_lo > 0 && (prm->M_lo < 2 || prm->M_lo >= prm->M)) || !(prm->lo_tol >= 0))
+    return set_err(c, LTB_ERR_ARG, "bad two-level parameters (0 or 2 <= M_lo < M, lo_tol >= 0)");
   const int M = prm->M, K = prm->K, S = b->S;
+  const bool two = prm->M_lo > 0;
   auto t0 = std::chrono::steady_clock::now();
   int st0 = ensure_vectors(b);
   if (st0) return st0;
@@ -882,6 +957,27 @@ int ltb_batch_scc_run(ltb_batch* b, const ltb_scc_params* prm, const double* gue
     b->M = M;
     b->K = K;
   }
+  if (two && b->M_lo != prm->M_lo) {
+    const int L = prm->M_lo;
+    for (void** p : {(void**)&b->d_tot_lo, (void**)&b->d_g_lo, (void**)&b->d_w_lo}) {
+      if (*p) (void)hipFree(*p);
+      *p = nullptr;
+    }
+    LTB_HIP(c, hipMalloc(&b->d_tot_lo, sizeof(double) * (size_t)(L + 1) * S));
+    LTB_HIP(c, hipMalloc(&b->d_g_lo, sizeof(double) * L));
+    LTB_HIP(c, hipMalloc(&b->d_w_lo, sizeof(double) * (size_t)L * S));
+    std::vector<double> g;
+    kernel_gmM(L, g);
+    LTB_HIP(c, hipMemcpy(b->d_g_lo, g.data(), sizeof(double) * L, hipMemcpyHostToDevice));
+    b->M_lo = L;
+  }
+  if (two && !b->d_masks) {
+    int st1;
+    if ((st1 = bmalloc(b, &b->d_masks, sizeof(int) * 3 * (size_t)S)) ||
+        (st1 = bmalloc(b, &b->d_plo, sizeof(double) * b->nat)) ||
+        (st1 = bmalloc(b, &b->d_defect, sizeof(double) * b->nat)))
+      return st1;
+  }
   if (guess)
     LTB_HIP(c, hipMemcpy(b->d_pin, guess, sizeof(double) * b->nat, hipMemcpyHostToDevice));
   else
@@ -889,46 +985,73 @@ int ltb_batch_scc_run(ltb_batch* b, const ltb_scc_params* prm, const double* gue
   // per structure: 1 recursing in FP32, 2 in FP64, 0 done (converged or failed)
   const int start = prm->precision == LTB_FP64 ? 2 : 1;
   std::vector<int> active(S, start), it_s(S, 0), st_s(S, LTB_NOT_CONVERGED);
+  // two-level, per structure in FP32: the phase (true: low-order iterations)
+  // and its stall test (as ltb_scc_run); a structure whose full-order |dq|
+  // falls below switch_tol goes on in plain FP64 iterations
+  std::vector<char> low(S, 0);
+  std::vector<int> plen(S, 0), stall(S, 0);
+  std::vector<double> best(S, 0.0);
+  const double lo_tol = prm->lo_tol > 0 ? prm->lo_tol : 0.1 * prm->tol;
+  std::vector<int> masks(3 * (size_t)S, 0), act_full(S, 0);
   std::vector<double> sc((size_t)S * NSC);
-  int it = 0, n64 = 0, ngraph = 0;
+  int it = 0, n64 = 0, ngraph = 0, nlo_it = 0;
   b->stat_launches = 0;
   b->stat_seconds = b->stat_bytes = 0;
-  GraphExecs<4> graphs;  // by the set of precisions in use (bit 0 FP32, bit 1 FP64)
+  // by the precisions in use up to step M_lo/2 (bits 0-1: FP32, FP64) and past it (bits 2-3)
+  GraphExecs<16> graphs;
   hipGraphExec_t* exec = graphs.e;
-  bool warmed[4] = {false, false, false, false};
+  bool warmed[16] = {};
   int result = LTB_OK;
+  const int nsteps = (M + 1) / 2, nlo = two ? (prm->M_lo + 1) / 2 : nsteps;
   for (it = 0; it < prm->max_iter; it++) {
-    int precs = 0;
-    for (int s = 0; s < S; s++) precs |= active[s] == 1 ? 1 : (active[s] == 2 ? 2 : 0);
+    int precs = 0, precs_full = 0;
+    for (int s = 0; s < S; s++) {
+      precs |= active[s] == 1 ? 1 : (active[s] == 2 ? 2 : 0);
+      act_full[s] = (two && active[s] == 1 && low[s]) ? 0 : active[s];
+      precs_full |= act_full[s] == 1 ? 1 : (act_full[s] == 2 ? 2 : 0);
+    }
     if (!precs) break;
+    if (!two) precs_full = 0;
+    const int key = precs | (precs_full << 2);
     LTB_HIP(c, hipMemcpy(b->d_active, active.data(), sizeof(int) * S, hipMemcpyHostToDevice));
-    if (prm->use_graph && warmed[precs]) {
-      if (!exec[precs]) {
+    if (two) {
+      for (int s = 0; s < S; s++) {
+        const bool two_s = active[s] == 1;  // FP32 structures run the two-level protocol
+        masks[s] = act_full[s];
+        masks[S + s] = two_s && low[s] ? 1 : 0;
+        masks[2 * S + s] = two_s ? (low[s] ? 1 : 2) : 0;
+      }
+      LTB_HIP(c, hipMemcpy(b->d_masks, masks.data(), sizeof(int) * masks.size(), hipMemcpyHostToDevice));
+      if (!precs_full) nlo_it++;
+    }
+    if (prm->use_graph && warmed[key]) {
+      if (!exec[key]) {
         hipGraph_t g = nullptr;
         LTB_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        const int s2 = batch_iteration(b, prm, precs, true);
+        const int s2 = batch_iteration(b, prm, precs, precs_full, true);
         const hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (s2) return s2;
         if (e != hipSuccess) return set_err(c, LTB_ERR_DEVICE, std::string("batch graph capture: ") + hipGetErrorString(e));
-        const hipError_t e2 = hipGraphInstantiate(&exec[precs], g, nullptr, nullptr, 0);
+        const hipError_t e2 = hipGraphInstantiate(&exec[key], g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (e2 != hipSuccess) return set_err(c, LTB_ERR_DEVICE, "batch graph instantiate failed");
       }
-      LTB_HIP(c, hipGraphLaunch(exec[precs], c->stream));
+      LTB_HIP(c, hipGraphLaunch(exec[key], c->stream));
       ngraph++;
     } else {
-      const int s2 = batch_iteration(b, prm, precs, false);
+      const int s2 = batch_iteration(b, prm, precs, precs_full, false);
       if (s2) return s2;
-      warmed[precs] = true;
+      warmed[key] = true;
       if (c->timing) {
         LTB_HIP(c, hipStreamSynchronize(c->stream));
-        for (int n = 0; n < (M + 1) / 2; n++) {
+        const int nlast = (!two || precs_full) ? nsteps : nlo;
+        for (int n = 0; n < nlast; n++) {
           float ms = 0;
           LTB_HIP(c, hipEventElapsedTime(&ms, b->events[2 * n], b->events[2 * n + 1]));
           b->stat_seconds += ms * 1e-3;
         }
-        b->stat_launches += (M + 1) / 2;
-        b->stat_bytes += iteration_bytes(b, active, M);
+        b->stat_launches += nlast;
+        b->stat_bytes += iteration_bytes(b, active, 0, nlo) + iteration_bytes(b, act_full, nlo, nsteps);
       }
     }
     if (precs & 2) n64++;
@@ -943,6 +1066,34 @@ int ltb_batch_scc_run(ltb_batch* b, const ltb_scc_params* prm, const double* gue
         st_s[s] = LTB_ERR_NONFINITE;
         active[s] = 0;
         result = LTB_ERR_NONFINITE;
+        continue;
+      }
+      if (two && active[s] == 1) {
+        if (low[s]) {  // the end of a low-order phase, as in ltb_scc_run
+          plen[s]++;
+          if (plen[s] == 1 || p[S_DQ] < 0.9 * best[s]) {
+            best[s] = p[S_DQ];
+            stall[s] = 0;
+          } else {
+            stall[s]++;
+          }
+          if (p[S_DQ] < lo_tol || stall[s] >= 4 || plen[s] >= 400) low[s] = 0;
+          n32++;
+          continue;
+        }
+        if (p[S_DQ] < prm->tol && prm->switch_tol <= 0) {
+          st_s[s] = LTB_OK;
+          active[s] = 0;
+          continue;
+        }
+        if (prm->switch_tol > 0 && p[S_DQ] < prm->switch_tol) {
+          active[s] = 2;  // the plain FP64 finish
+          n64s++;
+        } else {
+          low[s] = 1;
+          plen[s] = 0;
+          n32++;
+        }
         continue;
       }
       // the switch rule of ltb_scc_run, per structure
@@ -980,7 +1131,7 @@ int ltb_batch_scc_run(ltb_batch* b, const ltb_scc_params* prm, const double* gue
     total->eband_fp = 0;
     total->graph_launches = ngraph;
     total->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    total->lo_iterations = 0;
+    total->lo_iterations = nlo_it;
   }
   if (result) return set_err(c, result, "non-finite value in a batched structure");
   if (nconv < S) return set_err(c, LTB_NOT_CONVERGED, std::to_string(S - nconv) + " structures not converged");

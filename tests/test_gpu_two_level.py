@@ -15,6 +15,8 @@ mixing in place of updateHtotal).
   early full-order iterations, FP64 full-order iterations once |dq| < 1e-2 e,
   from the neutral start, against the plain FP64 solve: every charge within
   1e-6 e and the fixed-point band energy within 1e-6 eV/atom.
+* Batches (ltb_batch_scc_run with M_lo > 0): five uneven cells and the
+  256-cell workload against each cell's own plain FP64 solve.
 """
 import numpy as np
 import pytest
@@ -92,12 +94,66 @@ def test_two_level_parameter_checks(ctx, c0):
         assert e.value.status == L.LTB_ERR_ARG
     b = L.Batch(ctx, [s])
     try:
-        prm = L.SccParams(200, 0.05 / EV, 8192, 0.2, 1e-6, 10, L.FP64, 0.0, 0, 0, 32, 0.0)
-        z = np.zeros(s.nat)
-        zi = np.zeros(1, np.int32)
-        zd = np.zeros(1)
-        st = L.lib().ltb_batch_scc_run(b._h, L.C.byref(prm), None, L._d(z), L._i(zi), L._d(zd), L._d(zd),
-                                       L._i(zi), None)
-        assert st == L.LTB_ERR_ARG
+        for kw in (dict(M_lo=200), dict(M_lo=1), dict(M_lo=32, lo_tol=-1.0)):
+            with pytest.raises(L.LtbError) as e:
+                b.scc(200, precision=L.FP32_ACC64, switch_tol=1e-6, **kw)
+            assert e.value.status == L.LTB_ERR_ARG
     finally:
         b.close()
+
+
+# uneven cells (L, M, N, sigma A, seed), 72 to 162 atoms
+CELLS = [(2, 2, 2, 0.05, 1), (3, 2, 2, 0.10, 7), (2, 3, 2, 0.02, 11), (2, 2, 3, 0.15, 3),
+         (3, 3, 2, 0.08, 5)]
+
+
+def test_batch_two_level_matches_single_fp64_solves(ctx):
+    """Batched two-level SCC (FP32 structures: low-order iterations at M_lo =
+    32 between full-order ones, then plain FP64 iterations once a full-order
+    |dq| < 1e-6 e): every structure within 1e-6 e and 1e-6 eV/atom of its own
+    plain FP64 solve; graph replays equal eager iterations bit for bit."""
+    systems = [L.System.quartz(l, m, n, sigma_ang=sg, seed=sd) for l, m, n, sg, sd in CELLS]
+    singles = []
+    for s in systems:
+        ctx.load(s)
+        singles.append(ctx.scc(200, precision=L.FP64, tol=1e-6, max_iter=300))
+    with L.Batch(ctx, systems) as b:
+        kw = dict(precision=L.FP32_ACC64, switch_tol=1e-6, tol=1e-6, max_iter=2000, M_lo=32)
+        r = b.scc(200, use_graph=1, **kw)
+        e = b.scc(200, use_graph=0, **kw)
+    print(f"batch two-level: {r['batch_iters']} batch iterations ({r['lo_iters']} low-order only, "
+          f"{r['fp64_iters']} with FP64), per structure {list(r['iters'])}")
+    assert r["status"] == 0 and (r["struct_status"] == 0).all()
+    assert r["lo_iters"] > 0 and r["fp64_iters"] >= 1 and r["graph_launches"] > 0
+    for k, (q, g) in enumerate(zip(b.split(r["q"]), singles)):
+        assert np.abs(q - g["q"]).max() < 1e-6, k
+        assert abs(r["eband_fp"][k] - g["eband_fp"]) * EV / systems[k].nat < 1e-6, k
+        assert q.sum() == pytest.approx(systems[k].nelectrons, abs=1e-6)
+    assert np.array_equal(r["iters"], e["iters"]) and np.array_equal(r["q"], e["q"])
+    assert np.array_equal(r["eband_fp"], e["eband_fp"])
+
+
+def test_batch_two_level_of_256_cells(ctx):
+    """The 256-cell batched workload (86,211 atoms, M = 256) with the two-level
+    SCC: every cell converges and conserves its electrons; eight cells across
+    the size range match their own plain FP64 solves to 1e-6 e and 1e-6
+    eV/atom."""
+    from largetb_amd.learned import batch_structures
+
+    cells = batch_structures(256, seed=1)
+    systems = [L.System.quartz(l, m, n, sigma_ang=sg, seed=sd) for l, m, n, sg, sd in cells]
+    with L.Batch(ctx, systems) as b:
+        r = b.scc(256, precision=L.FP32_ACC64, switch_tol=1e-6, tol=1e-6, max_iter=2000, use_graph=1, M_lo=32)
+        qs = b.split(r["q"])
+    print(f"256 cells two-level: {r['batch_iters']} batch iterations ({r['lo_iters']} low-order only), "
+          f"{r['seconds']:.2f} s")
+    assert r["status"] == 0 and (r["struct_status"] == 0).all()
+    for k, s in enumerate(systems):
+        assert qs[k].sum() == pytest.approx(s.nelectrons, abs=1e-6)
+    sizes = np.array([s.nat for s in systems])
+    order = np.argsort(sizes, kind="stable")
+    for k in order[np.linspace(0, len(order) - 1, 8).round().astype(int)]:
+        ctx.load(systems[k])
+        g = ctx.scc(256, precision=L.FP64, tol=1e-6, max_iter=400, use_graph=1)
+        assert np.abs(qs[k] - g["q"]).max() < 1e-6, k
+        assert abs(r["eband_fp"][k] - g["eband_fp"]) * EV / systems[k].nat < 1e-6, k
