@@ -4,8 +4,9 @@ The training structures follow the batched workload: alpha-quartz supercells
 with L, M in {2..5}, N in {2..4} and a displacement sigma uniform in
 [0.02, 0.15] A, drawn from a SplitMix64 stream.  Each is solved from the
 neutral start without any guess: either one at a time by the FP64 SCC
-(`generate`) or 256 at a time by the batched SCC with an FP32 bulk and FP64
-finish (`generate_batched`, charges within ~1e-8 e of the FP64 solve).  The
+(`generate`) or 256 at a time by the batched SCC (`solve_batched`: the
+two-level SCC with low-order iterations at M_lo = 32, an FP32 bulk and an FP64
+finish; charges within ~3e-8 e of each structure's FP64 solve).  The
 descriptors and converged charges form the data.  Training runs the device
 kernels (ltb_mlp_adam).
 """
@@ -70,9 +71,10 @@ def generate(ctx, structures, M=512, kT_ev=0.05, log=None):
                 seg=np.array(seg, np.int32), qtot=np.array(qtot), meta=meta)
 
 
-def solve_batched(ctx, structures, M=256, kT_ev=0.05, per_batch=256, log=None):
+def solve_batched(ctx, structures, M=256, kT_ev=0.05, per_batch=256, log=None, M_lo=32):
     """Converged charges of every structure, `per_batch` structures per batched
-    SCC (FP32 bulk, FP64 finish, |dq| < 1e-6 e).  Returns the per-atom charges
+    SCC (FP32 bulk, FP64 finish, |dq| < 1e-6 e; with M_lo > 0 the bulk is the
+    two-level SCC, low-order iterations at M_lo).  Returns the per-atom charges
     (e, structure order) and per-structure atom offsets, iterations, status."""
     qs, offs, iters, status = [], [0], [], []
     t0 = time.perf_counter()
@@ -82,8 +84,8 @@ def solve_batched(ctx, structures, M=256, kT_ev=0.05, per_batch=256, log=None):
         with Batch(ctx, systems) as b:
             if b0 == 0:
                 b.autotune(FP32_ACC64, 3)
-            r = b.scc(M, kT_ev=kT_ev, precision=FP32_ACC64, switch_tol=1e-6, tol=1e-6, max_iter=400,
-                      use_graph=1, check=False)
+            r = b.scc(M, kT_ev=kT_ev, precision=FP32_ACC64, switch_tol=1e-6, tol=1e-6,
+                      max_iter=2000 if M_lo else 400, use_graph=1, check=False, M_lo=M_lo)
         qs.append(r["q"])
         for s in systems:
             offs.append(offs[-1] + s.nat)
@@ -154,6 +156,10 @@ def load(ctx, path=WEIGHTS):
 def main(argv=None):
     """python -m largetb_amd.learned [--count 256] [--seed 2026] [--M 512] [--steps 3000]
     generates the training set on the GPU, trains, and writes guess_weights.npz.
+    The shipped guess_weights.npz was made with --batched --M 256 over 20,480
+    structures (two generation runs of 10,240 with --charges-out, then
+    --charges-in both --init guess_weights_256.npz --steps 1700);
+    guess_weights_256.npz is the default recipe's 256-structure network.
     --batched solves 256 structures per batched SCC; --charges-out writes only
     the converged charges (with --skip-train, a generation-only run);
     --charges-in trains from charges files written that way (several files =
