@@ -55,6 +55,9 @@ struct Settings {
                                         // mixRatio (Hamiltonian mixing equals charge mixing, Gamma linear)
   std::string dump_dir;                 // non-empty: write the reference's MATLAB dumps there (H.m,
                                         // GAMMA_AB.m, deltaH.m, netCharge.m, GammaQ_vector.m)
+  int two_level_order = 0;              // > 0 (exact trace, maxsp2 1): scf runs the device two-level SCC
+                                        // (ltb_scc_run with M_lo = this order; FP32 full-order iterations
+                                        // until |dq| < 1e-2 e with --fp32, FP64 after)
 };
 
 struct State {

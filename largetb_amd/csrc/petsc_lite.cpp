@@ -411,8 +411,48 @@ PetscErrorCode scf_literal(State& s, int* iterations, double* delta) {
   return d < s.opt.tol ? LTB_OK : LTB_NOT_CONVERGED;
 }
 
+// the same loop as scf on the device (ltb_scc_run) with the two-level SCC:
+// most iterations at order two_level_order, corrected by the defect of the
+// last full-order one; converged only on a full-order iteration
+PetscErrorCode scf_two_level(State& s, int* iterations, double* delta) {
+  if (s.opt.nrandoms > 0 || s.opt.maxsp2 > 1 || s.opt.two_level_order >= s.opt.moments || s.opt.kT_ev <= 0) {
+    s.last_error = "the two-level SCC needs exact trace, maxsp2 1, Fermi-Dirac kT > 0 and an order below M";
+    return LTB_ERR_ARG;
+  }
+  ltb_scc_params p{};
+  p.M = s.opt.moments;
+  p.kT = s.opt.kT_ev / 27.211386;
+  p.K = s.opt.quad_nodes;
+  p.mix = s.opt.mix;
+  p.tol = s.opt.tol;
+  p.max_iter = s.opt.max_scf;
+  p.precision = s.opt.precision;
+  p.switch_tol = s.opt.precision == LTB_FP32_ACC64 ? 1e-2 : 0.0;
+  p.use_graph = 1;
+  p.M_lo = s.opt.two_level_order;
+  std::vector<double> pop(s.Natoms), hist(s.opt.max_scf);
+  ltb_scc_result r{};
+  const int st = ltb_scc_run(s.ctx, &p, s.mixedCharges.data(), pop.data(), hist.data(), &r);
+  if (st && st != LTB_NOT_CONVERGED) {
+    s.last_error = ltb_last_error(s.ctx);
+    return st;
+  }
+  if (s.opt.verbose)
+    for (int it = 0; it < r.iterations; it++)
+      std::printf("scfiter : %d , the value of the scfDelta is %e \n", it, hist[it]);  // :1835
+  s.chargeOnAtom = pop;
+  CHK(s, ltb_get_mixed_populations(s.ctx, s.mixedCharges.data()));
+  s.previousCharges = s.chargeOnAtom;
+  s.mu_ha = r.mu_ha;
+  s.eband = r.eband;
+  if (iterations) *iterations = r.iterations;
+  if (delta) *delta = r.dq;
+  return st;
+}
+
 PetscErrorCode scf(State& s, int* iterations, double* delta) {
   if (s.opt.reference_literal) return scf_literal(s, iterations, delta);
+  if (s.opt.two_level_order > 0) return scf_two_level(s, iterations, delta);
   int it = 0;
   double d = INFINITY;
   for (it = 0; it < s.opt.max_scf; it++) {

@@ -5,6 +5,7 @@
 //   ltb_scf L M N [nRandoms] [--sigma A] [--seed S] [--moments M] [--kT eV] [--reference]
 //                 [--maxsp2 D] [--tol Q] [--max-scf K] [--fp32] [--sp2-alpha a] [--dump DIR]
 //                 [--mix m] [--reference-literal] [--sp2-alphas a1,a2,..] [--kpe-times-two]
+//                 [--two-level M_lo]
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -23,7 +24,7 @@ int main(int argc, char** argv) {
   if (argc < 4) {
     std::fprintf(stderr, "usage: %s L M N [nRandoms] [--sigma A] [--seed S] [--moments M] [--kT eV] "
                  "[--reference] [--maxsp2 D] [--tol Q] [--max-scf K] [--fp32] [--sp2-alpha a] [--sp2-alphas a1,a2,..] [--kpe-times-two] [--dump DIR] [--mix m] "
-                 "[--reference-literal]\n", argv[0]);
+                 "[--reference-literal] [--two-level M_lo]\n", argv[0]);
     return 2;
   }
   State s;
@@ -59,6 +60,7 @@ int main(int argc, char** argv) {
     else if (a == "--dump") s.opt.dump_dir = val();
     else if (a == "--mix") s.opt.mix = std::atof(val());
     else if (a == "--reference-literal") s.opt.reference_literal = true;
+    else if (a == "--two-level") s.opt.two_level_order = std::atoi(val());
     else {
       std::fprintf(stderr, "unknown option %s\n", a.c_str());
       return 2;

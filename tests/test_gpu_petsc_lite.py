@@ -44,6 +44,26 @@ def test_scf_driver_matches_oracle(tmp_path):
     assert out["total_charge"] == pytest.approx(384.0, abs=1e-8)
 
 
+@pytest.mark.gpu
+def test_scf_driver_two_level(tmp_path):
+    """ltb_scf --two-level 32: the same fixed point as the oracle's plain SCC
+    (the two-level SCC converges only on a full-order iteration), with the
+    reference's per-iteration lines, in FP64 and with --fp32."""
+    o = O.System.quartz(2, 2, 2, sigma_ang=0.05, seed=1)
+    ref = o.scc(200, tol=1e-10, maxit=600)
+    for extra in ([], ["--fp32"]):
+        r, out, q = run(["2", "2", "2", "--sigma", "0.05", "--seed", "1", "--moments", "200", "--max-scf", "1000",
+                         "--two-level", "32"] + extra, tmp_path)
+        assert r.returncode == 0, r.stderr
+        assert out["status"] == 0 and out["iterations"] > 100
+        assert r.stdout.count("scfiter : ") == out["iterations"]
+        assert np.abs(q - ref["q"]).max() < 1e-6
+        assert out["total_charge"] == pytest.approx(384.0, abs=1e-8)
+    r = subprocess.run([DRIVER, "2", "2", "2", "--moments", "200", "--two-level", "200"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 1 and "two-level" in r.stderr
+
+
 def p0_eigen(M=200):
     o = O.System.quartz(2, 2, 2, sigma_ang=0.05, seed=1)
     d = o.density(M)

@@ -157,3 +157,35 @@ def test_batch_two_level_of_256_cells(ctx):
         g = ctx.scc(256, precision=L.FP64, tol=1e-6, max_iter=400, use_graph=1)
         assert np.abs(qs[k] - g["q"]).max() < 1e-6, k
         assert abs(r["eband_fp"][k] - g["eband_fp"]) * EV / systems[k].nat < 1e-6, k
+
+
+def test_c3_learned_two_level_reaches_the_plain_fixed_point(ctx):
+    """The headline workload (12x12x12, 15,552 atoms, sigma 0.10 A, seed 1,
+    M = 512) as the benchmark runs it: the shipped learned guess, then the
+    two-level SCC (M_lo = 32; FP32 full-order iterations until |dq| < 1e-2 e,
+    FP64 after) converges in at most 4 full-order iterations.  One plain FP64
+    iteration started from its answer moves every charge by < 1e-7 e and the
+    band energy by < 1e-7 eV/atom, so the answer is the fixed point the plain
+    FP64 solve converges to (that solve stops within ~1e-8 e of it)."""
+    from largetb_amd import learned
+    s = L.System.quartz(12, 12, 12, sigma_ang=0.10, seed=1)
+    ctx.load(s)
+    ctx.set_populations(None)
+    lo, hi = ctx.gershgorin()
+    ctx.rescale((hi - lo) / 2, (hi + lo) / 2)
+    for prec in (L.FP64, L.FP32_ACC64):
+        ctx.autotune(prec, 3)
+    m, _ = learned.load(ctx)
+    guess = m.predict(s)
+    m.close()
+    r = ctx.scc(512, guess=guess, precision=L.FP32_ACC64, switch_tol=1e-2, tol=1e-6, max_iter=2000, use_graph=1,
+                M_lo=32)
+    chk = ctx.scc(512, guess=r["pop"], precision=L.FP64, tol=1e-30, max_iter=1, check=False)
+    full = r["iters"] - r["lo_iters"]
+    deb = abs(chk["eband_fp"] - r["eband_fp"]) * EV / s.nat
+    print(f"15,552 atoms: learned guess off by {np.abs(2 * guess - r['q']).max():.2e} e; two-level {full} full-order "
+          f"({r['fp64_iters']} FP64) + {r['lo_iters']} low-order, {r['seconds']:.2f} s; plain FP64 iteration from the "
+          f"answer: |dq| {chk['hist'][0]:.2e} e, |dE_band| {deb:.2e} eV/atom")
+    assert r["status"] == 0 and full <= 4 and r["fp64_iters"] >= 1
+    assert chk["hist"][0] < 1e-7 and deb < 1e-7
+    assert r["q"].sum() == pytest.approx(s.nelectrons, abs=1e-6)
