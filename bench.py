@@ -10,12 +10,13 @@ value = algorithmic Chebyshev block-SpMM bytes of the timed iterations divided
 by their wall time (GB/s).  roofline = the block-SpMM kernel's algorithmic
 bytes per launch over its mean launch time (HIP events on the context stream).
 time_to_solution (default on, --no-tts): plain FP64 SCC from the neutral
-start against the learned guess with FP32 bulk and FP64 finish, and against
-the learned guess with the two-level SCC (low-order iterations at M=32
-corrected by the defect of the last full-order iteration, FP32 full-order
-iterations until |dq| < 1e-2 e, FP64 after), all to |dq| < 1e-6 e on a
-full-order iteration, within --time-budget seconds of the start (else it says
-it was skipped); one line per SCC iteration goes to stderr meanwhile.
+start against the learned guess with the two-level SCC (low-order iterations
+at M=32 corrected by the defect of the last full-order iteration, FP32
+full-order iterations until |dq| < 1e-2 e, FP64 after), then, as the time
+budget allows, against the learned guess with FP32 bulk and FP64 finish; all
+to |dq| < 1e-6 e on a full-order FP64 iteration, within --time-budget seconds
+of the start (what would not fit is skipped, saying so); one line per SCC
+iteration goes to stderr meanwhile.
 """
 import argparse
 import json
@@ -279,13 +280,14 @@ def two_level_check(args):
 
 
 def tts_leg(ctx, s, args, t32_iter):
-    """Time to solution on the bench structure, both runs to |dq| < 1e-6 e from
-    inputs resident on the device: plain FP64 from the neutral start, and the
-    learned guess (descriptors + network inference inside the clock) with FP32
-    bulk iterations and an FP64 finish.  Also the FP64 Chebyshev kernel's rate.
-    The leg checks the remaining time budget first and skips itself (saying
-    so) rather than run past it; the plain run's iteration cap follows from the
-    budget too."""
+    """Time to solution on the bench structure, every run to |dq| < 1e-6 e from
+    inputs resident on the device: plain FP64 from the neutral start; the
+    learned guess (descriptors + network inference inside the clock) with the
+    two-level SCC; then, if the time budget still allows, the learned guess
+    with FP32 bulk iterations and an FP64 finish.  Also the FP64 Chebyshev
+    kernel's rate.  The leg checks the remaining time budget before each run
+    and skips what would not fit (saying so); the plain run's iteration cap
+    follows from the budget too."""
     import numpy as np
     import largetb_amd as L
     from largetb_amd import learned
@@ -305,58 +307,62 @@ def tts_leg(ctx, s, args, t32_iter):
     out["k1_fp64"] = {"achieved": round(k1, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                       "frac": round(k1 / HBM_PEAK_GBS, 4), "launches": st["launches"],
                       "mean_launch_us": round(st["seconds"] / max(st["launches"], 1) * 1e6, 2), "tiling": t64}
+    ctx.scc(args.M, precision=L.FP32_ACC64, max_iter=2, tol=1e-30, check=False, M_lo=TWO_LEVEL_M_LO)  # warm-up
     m, info = learned.load(ctx, args.weights) if args.weights else learned.load(ctx)
-    # iteration counts measured on this workload: 73 plain, 39 learned + mixed,
-    # and the two-level run's ~3 full-order + ~80 low-order ones
-    two_est = 2 * t64_iter + t32_iter + 90 * t32_iter * TWO_LEVEL_M_LO / args.M + 2.0
-    est = 1.05 * (76 * t64_iter + 42 * t32_iter + 2.0 + two_est)
+    # iteration counts measured on this workload: 73 plain; two-level ~3
+    # full-order + ~90 low-order; 40 learned + mixed
+    two_est = 1.05 * (2 * t64_iter + t32_iter + 95 * t32_iter * TWO_LEVEL_M_LO / args.M + 2.0)
+    plain_est = 1.05 * 76 * t64_iter
+    mixed_est = 1.05 * (43 * t32_iter + 2.0)
     left = args.time_budget - (time.perf_counter() - T_START)
-    out["estimate_s"] = round(est, 1)
-    if est > left:
-        out["skipped"] = f"estimated {est:.0f} s exceeds the {left:.0f} s left of the {args.time_budget:.0f} s budget"
+    out["estimate_s"] = round(plain_est + two_est + mixed_est, 1)
+    if plain_est + two_est > left:
+        out["skipped"] = (f"estimated {plain_est + two_est:.0f} s (plain + two-level) exceeds the {left:.0f} s left "
+                          f"of the {args.time_budget:.0f} s budget")
         m.close()
         return out
     log("time to solution: plain FP64 from the neutral start")
-    cap = int((left - 1.05 * (42 * t32_iter + 2.0 + two_est)) / t64_iter)
+    cap = int((left - two_est) / (1.02 * t64_iter))
     t0 = time.perf_counter()
     r = ctx.scc(args.M, precision=L.FP64, tol=1e-6, max_iter=min(400, cap), use_graph=1, check=False, verbose=1)
     plain = dict(seconds=time.perf_counter() - t0, iters=r["iters"], status=r["status"])
     qp, ep, mp = r["q"], r["eband_fp"], r["mu_ha"]
+    out["plain_fp64"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in plain.items()}
     if r["status"] != 0:
-        out["plain_fp64"] = plain
         out["skipped"] = f"plain FP64 stopped at its budget cap of {cap} iterations"
         m.close()
         return out
-    log("time to solution: learned guess + FP32 bulk + FP64 finish")
-    left = args.time_budget - (time.perf_counter() - T_START)
-    t0 = time.perf_counter()
-    guess = m.predict(s)
-    tpred = time.perf_counter() - t0
-    r = ctx.scc(args.M, guess=guess, precision=L.FP32_ACC64, switch_tol=1e-6, tol=1e-6,
-                max_iter=max(1, min(400, int((left - two_est) / t32_iter) - 2)), use_graph=1, check=False,
-                verbose=1)
-    fast = dict(seconds=time.perf_counter() - t0, iters=r["iters"], fp64_iters=r["fp64_iters"],
-                status=r["status"], predict_s=round(tpred, 3),
-                guess_err_e=float(np.abs(2 * guess - qp).max()))
-    rf = r
+
+    def record(name, d, rr):
+        out[name] = {k: (round(v, 3) if isinstance(v, float) and k != "guess_err_e" else v) for k, v in d.items()}
+        out[name]["max_dq_vs_plain_e"] = float(np.abs(rr["q"] - qp).max())
+        out[name]["deband_vs_plain_ev_per_atom"] = abs(rr["eband_fp"] - ep) * EV / s.nat
+        out[name]["dmu_vs_plain_ev"] = abs(rr["mu_ha"] - mp) * EV
+        out[f"speedup_{name}_vs_plain_fp64"] = round(plain["seconds"] / d["seconds"], 3)
+
     log(f"time to solution: learned guess + two-level SCC (M_lo={TWO_LEVEL_M_LO}, FP32 until |dq| < 1e-2 e)")
     t0 = time.perf_counter()
     guess = m.predict(s)
     tpred = time.perf_counter() - t0
     r = ctx.scc(args.M, guess=guess, precision=L.FP32_ACC64, switch_tol=1e-2, tol=1e-6, max_iter=2000,
                 use_graph=1, check=False, verbose=1, M_lo=TWO_LEVEL_M_LO)
-    two = dict(seconds=time.perf_counter() - t0, iters=r["iters"], full_order_iters=r["iters"] - r["lo_iters"],
-               fp64_iters=r["fp64_iters"], lo_iters=r["lo_iters"], M_lo=TWO_LEVEL_M_LO, status=r["status"],
-               predict_s=round(tpred, 3))
+    record("learned_mixed_two_level",
+           dict(seconds=time.perf_counter() - t0, iters=r["iters"], full_order_iters=r["iters"] - r["lo_iters"],
+                fp64_iters=r["fp64_iters"], lo_iters=r["lo_iters"], M_lo=TWO_LEVEL_M_LO, status=r["status"],
+                predict_s=round(tpred, 3), guess_err_e=float(np.abs(2 * guess - qp).max())), r)
+    left = args.time_budget - (time.perf_counter() - T_START)
+    if mixed_est > left:
+        out["learned_mixed"] = {"skipped": f"estimated {mixed_est:.0f} s exceeds the {left:.0f} s left"}
+    else:
+        log("time to solution: learned guess + FP32 bulk + FP64 finish")
+        t0 = time.perf_counter()
+        guess = m.predict(s)
+        tpred = time.perf_counter() - t0
+        r = ctx.scc(args.M, guess=guess, precision=L.FP32_ACC64, switch_tol=1e-6, tol=1e-6,
+                    max_iter=max(1, min(400, int(left / t32_iter) - 2)), use_graph=1, check=False, verbose=1)
+        record("learned_mixed", dict(seconds=time.perf_counter() - t0, iters=r["iters"], fp64_iters=r["fp64_iters"],
+                                     status=r["status"], predict_s=round(tpred, 3)), r)
     m.close()
-    out["plain_fp64"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in plain.items()}
-    for name, d, rr in (("learned_mixed", fast, rf), ("learned_mixed_two_level", two, r)):
-        out[name] = {k: (round(v, 3) if isinstance(v, float) and k != "guess_err_e" else v) for k, v in d.items()}
-        out[name]["max_dq_vs_plain_e"] = float(np.abs(rr["q"] - qp).max())
-        out[name]["deband_vs_plain_ev_per_atom"] = abs(rr["eband_fp"] - ep) * EV / s.nat
-        out[name]["dmu_vs_plain_ev"] = abs(rr["mu_ha"] - mp) * EV
-    out["speedup_learned_mixed_vs_plain_fp64"] = round(plain["seconds"] / fast["seconds"], 3)
-    out["speedup_learned_mixed_two_level_vs_plain_fp64"] = round(plain["seconds"] / two["seconds"], 3)
     out["weights"] = info
     return out
 
