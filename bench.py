@@ -421,9 +421,11 @@ def c1_measure(args):
                                "max_dq_e": float(2 * np.abs(sp["fp64"]["pop"] - sp["fp32_fp64tail"]["pop"]).max())}
     out["sp2_fp64_vs_kpe_fp64_max_dq_e"] = float(2 * np.abs(sp["fp64"]["pop"] - a["pop"]).max())
     runs = {}
+    ctx.scc(512, precision=L.FP32_ACC64, max_iter=2, tol=1e-30, check=False, M_lo=TWO_LEVEL_M_LO)  # warm-up
     for name, kw in (("plain_fp64", dict(precision=L.FP64, max_iter=400)),
                      ("mixed", dict(precision=L.FP32_ACC64, switch_tol=1e-6, max_iter=400)),
-                     ("fp32_only", dict(precision=L.FP32_ACC64, max_iter=150))):
+                     ("fp32_only", dict(precision=L.FP32_ACC64, max_iter=150)),
+                     ("two_level", dict(precision=L.FP32_ACC64, switch_tol=1e-2, max_iter=2000, M_lo=TWO_LEVEL_M_LO))):
         t0 = time.perf_counter()
         r = ctx.scc(512, tol=1e-6, use_graph=1, check=False, **kw)
         runs[name] = dict(r, wall=time.perf_counter() - t0)
@@ -434,6 +436,8 @@ def c1_measure(args):
                               "max_dq_vs_plain_e": float(np.abs(r["q"] - ref["q"]).max()),
                               "deband_vs_plain_ev_per_atom": abs(r["eband_fp"] - ref["eband_fp"]) * EV / s.nat,
                               "dmu_vs_plain_ev": abs(r["mu_ha"] - ref["mu_ha"]) * EV}
+        if name == "two_level":
+            out[f"scc_{name}"].update(lo_iters=r["lo_iters"], M_lo=TWO_LEVEL_M_LO, start="neutral")
     out["command"] = "python3 -u bench.py --c1"
     print(json.dumps(out))
     ctx.close()

@@ -360,12 +360,13 @@ int grid_for(int n, int bs) { return (n + bs - 1) / bs; }
 
 // Fermi quadrature / kernel setup for (M, K)
 int ensure_fermi(ltb_ctx* c, int M, int K) {
-  if (c->fd_M == M && c->fd_K == K && c->d_W) return LTB_OK;
+  if (c->fd_M == M && c->fd_K == K && c->d_W && c->d_X && c->d_g && c->d_w) return LTB_OK;
   (void)hipFree(c->d_W);
   (void)hipFree(c->d_X);
   (void)hipFree(c->d_g);
   (void)hipFree(c->d_w);
   c->d_W = c->d_X = c->d_g = c->d_w = nullptr;
+  c->fd_M = c->fd_K = 0;  // a failed allocation below leaves no set that looks valid
   LTB_HIP(c, hipMalloc(&c->d_W, sizeof(double) * std::max(K, 1)));
   LTB_HIP(c, hipMalloc(&c->d_X, sizeof(double) * std::max(K, 1)));
   LTB_HIP(c, hipMalloc(&c->d_g, sizeof(double) * M));
