@@ -750,7 +750,7 @@ def main():
                     help="only run 1 GiB device-to-device copies (counter calibration)")
     ap.add_argument("--no-tts", action="store_true",
                     help="default run: leave out the time-to-solution leg")
-    ap.add_argument("--time-budget", type=float, default=570.0,
+    ap.add_argument("--time-budget", type=float, default=550.0,
                     help="default run: seconds since start within which the time-to-solution leg must end")
     args = ap.parse_args()
     if args.copy_calibration:
