@@ -171,6 +171,14 @@ int ltb_density(ltb_ctx* ctx, const ltb_fermi_params* fp, double* pop, ltb_densi
  * mode :1361-1379) at the mu of the last ltb_density call.  Needs per-orbital
  * moments: ltb_set_kpe_orbital_moments(ctx, 1) before ltb_kpe_moments. */
 int ltb_set_kpe_orbital_moments(ltb_ctx* ctx, int on);
+/* Light cones (default on; no counterpart in the reference, whose getKPE
+ * recursion :1148-1168 always runs over every row): with more than one
+ * unit-vector chunk under the sweep schedule, recursion step n >= 2 of a chunk
+ * visits only the atoms within n + 1 hops of the chunk's own atoms in H~'s
+ * block graph (a cyclic atom range per chunk and step), since every other row
+ * of t_{n+1} is zero.  Moments equal the full sweep's up to the order of the
+ * FP64 dot sums; ltb_kpe_stats counts the bytes of the visited rows only. */
+int ltb_set_kpe_light_cone(ltb_ctx* ctx, int on);
 /* the weights w[m] = c_m(mu) g_m (w[0] = c_0/2) of the last ltb_density call,
  * as getKPE applies them (:1134-1166) -- input of ltb_kpe_apply */
 int ltb_kpe_weights(ltb_ctx* ctx, int M, double* w);

@@ -132,6 +132,7 @@ SIGNATURES = {
     "ltb_sp2_dense": (C.c_int, [VP, C.c_double, C.c_int, C.c_int, PD, PD, C.POINTER(Sp2Result)]),
     "ltb_kpe_stats": (C.c_int, [VP, PI, PD, PD]),
     "ltb_set_kpe_orbital_moments": (C.c_int, [VP, C.c_int]),
+    "ltb_set_kpe_light_cone": (C.c_int, [VP, C.c_int]),
     "ltb_density_diag": (C.c_int, [VP, PD]),
     "ltb_get_orbital_moments": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, PD]),
     "ltb_kpe_weights": (C.c_int, [VP, C.c_int, PD]),
@@ -525,6 +526,10 @@ class Context:
 
     def keep_orbital_moments(self, on=True):
         self._check(lib().ltb_set_kpe_orbital_moments(self._h, 1 if on else 0))
+
+    def set_light_cone(self, on=True):
+        """Light-cone restriction of the unit-vector recursion (default on)."""
+        self._check(lib().ltb_set_kpe_light_cone(self._h, 1 if on else 0))
 
     def kpe_weights(self, M):
         w = np.zeros(M)

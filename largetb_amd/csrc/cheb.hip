@@ -145,7 +145,8 @@ __global__ __launch_bounds__(256) void k_cheb_sweep(const int* __restrict__ rowp
                                                     T* __restrict__ xnext, int nat, int ld, int j0,
                                                     int nvalid, int n_cg, int slots,
                                                     const int* __restrict__ visit,
-                                                    double* __restrict__ part, FoldArgs prev) {
+                                                    double* __restrict__ part, FoldArgs prev, int alo,
+                                                    int alen) {
   constexpr int W = 64 * V;
   // the previous step's partials (complete: it was the previous launch) are
   // folded here instead of in a launch of their own
@@ -161,9 +162,13 @@ __global__ __launch_bounds__(256) void k_cheb_sweep(const int* __restrict__ rowp
     double sxy[V], syy[V];
 #pragma unroll
     for (int v = 0; v < V; v++) sxy[v] = syy[v] = 0.0;
-    for (int i = wid; i < nat; i += S)
-      atom_step<T, V, MODE, NT>(rowptr, col, hv, xcur, xnext, visit ? visit[i] : i, ld, j0, c0, nvalid,
-                                sxy, syy);
+    // the atoms alo .. alo + alen - 1 (cyclic): the chunk's light cone, or all
+    for (int i = wid; i < alen; i += S) {
+      int A = alo + i;
+      A -= A >= nat ? nat : 0;
+      atom_step<T, V, MODE, NT>(rowptr, col, hv, xcur, xnext, visit ? visit[A] : A, ld, j0, c0, nvalid, sxy,
+                                syy);
+    }
     flush_partials<V>(sxy, syy, wave, lane, part, (size_t)(xcd * slots + slot) * 2, ld, cg * W);
   }
 }
@@ -224,6 +229,12 @@ __global__ __launch_bounds__(256) void k_block_reduce(const double* __restrict__
   if (2 * n + 2 <= M) momcol[(size_t)(2 * n + 2) * ld + c] = 2.0 * syy - r2;
 }
 
+// atoms a launch of the sweep schedule visits: alo .. alo + alen - 1
+// (cyclic); alen < 0: all of them
+struct AtomRange {
+  int lo = 0, len = -1;
+};
+
 struct Plan {
   int V, W, ld, schedule, nt, order;
   int apw, n_sg;   // blocked
@@ -271,12 +282,14 @@ Plan make_plan(const ltb_ctx* c) {
 // fold first (sweep schedule only; prev.part == nullptr for none)
 template <typename T, int V, int MODE, bool NT>
 void launch_one(const ltb_ctx* c, const T* hv, const T* xcur, T* xnext, int ldc, int j0, int nvalid,
-                const Plan& p, double* part, const FoldArgs& prev) {
+                const Plan& p, double* part, const FoldArgs& prev, AtomRange r) {
   const int n_cg = ldc / (64 * V);
   if (p.schedule == LTB_SCHED_SWEEP) {
+    const bool all = r.len < 0 || r.len >= c->nat;
     hipLaunchKernelGGL((k_cheb_sweep<T, V, MODE, NT>), dim3(8 * p.slots), dim3(256), 0, c->stream,
                        c->d_rowptr, c->d_col, hv, xcur, xnext, c->nat, ldc, j0, nvalid, n_cg,
-                       p.slots, p.order == 1 ? c->d_visit : nullptr, part, prev);
+                       p.slots, p.order == 1 ? c->d_visit : nullptr, part, prev, all ? 0 : r.lo,
+                       all ? c->nat : r.len);
   } else {
     hipLaunchKernelGGL((k_cheb_blocked<T, V, MODE, NT>), dim3(n_cg * p.n_sg), dim3(256), 0,
                        c->stream, c->d_rowptr, c->d_col, hv, xcur, xnext, c->nat, ldc, j0, nvalid,
@@ -286,35 +299,97 @@ void launch_one(const ltb_ctx* c, const T* hv, const T* xcur, T* xnext, int ldc,
 
 template <typename T, int V, int MODE>
 void launch_nt(const ltb_ctx* c, const T* hv, const T* xcur, T* xnext, int ldc, int j0, int nvalid,
-               const Plan& p, double* part, const FoldArgs& prev) {
+               const Plan& p, double* part, const FoldArgs& prev, AtomRange r) {
   if (p.nt)
-    launch_one<T, V, MODE, true>(c, hv, xcur, xnext, ldc, j0, nvalid, p, part, prev);
+    launch_one<T, V, MODE, true>(c, hv, xcur, xnext, ldc, j0, nvalid, p, part, prev, r);
   else
-    launch_one<T, V, MODE, false>(c, hv, xcur, xnext, ldc, j0, nvalid, p, part, prev);
+    launch_one<T, V, MODE, false>(c, hv, xcur, xnext, ldc, j0, nvalid, p, part, prev, r);
 }
 
 template <typename T, int V>
 void launch_mode(const ltb_ctx* c, int mode, const T* hv, const T* xcur, T* xnext, int ldc, int j0,
-                 int nvalid, const Plan& p, double* part, const FoldArgs& prev) {
+                 int nvalid, const Plan& p, double* part, const FoldArgs& prev, AtomRange r) {
   if (mode == 0)
-    launch_nt<T, V, 0>(c, hv, xcur, xnext, ldc, j0, nvalid, p, part, prev);
+    launch_nt<T, V, 0>(c, hv, xcur, xnext, ldc, j0, nvalid, p, part, prev, r);
   else if (mode == 1)
-    launch_nt<T, V, 1>(c, hv, xcur, xnext, ldc, j0, nvalid, p, part, prev);
+    launch_nt<T, V, 1>(c, hv, xcur, xnext, ldc, j0, nvalid, p, part, prev, r);
   else if (mode == 2)
-    launch_nt<T, V, 2>(c, hv, xcur, xnext, ldc, j0, nvalid, p, part, prev);
+    launch_nt<T, V, 2>(c, hv, xcur, xnext, ldc, j0, nvalid, p, part, prev, r);
   else
-    launch_nt<T, V, 3>(c, hv, xcur, xnext, ldc, j0, nvalid, p, part, prev);
+    launch_nt<T, V, 3>(c, hv, xcur, xnext, ldc, j0, nvalid, p, part, prev, r);
 }
 
 template <typename T>
 void launch_step(const ltb_ctx* c, int mode, const T* hv, const T* xcur, T* xnext, int ldc, int j0,
-                 int nvalid, const Plan& p, double* part, const FoldArgs& prev = FoldArgs{}) {
+                 int nvalid, const Plan& p, double* part, const FoldArgs& prev = FoldArgs{},
+                 AtomRange r = AtomRange{}) {
   if (p.V == 4)
-    launch_mode<T, 4>(c, mode, hv, xcur, xnext, ldc, j0, nvalid, p, part, prev);
+    launch_mode<T, 4>(c, mode, hv, xcur, xnext, ldc, j0, nvalid, p, part, prev, r);
   else if (p.V == 2)
-    launch_mode<T, 2>(c, mode, hv, xcur, xnext, ldc, j0, nvalid, p, part, prev);
+    launch_mode<T, 2>(c, mode, hv, xcur, xnext, ldc, j0, nvalid, p, part, prev, r);
   else
-    launch_mode<T, 1>(c, mode, hv, xcur, xnext, ldc, j0, nvalid, p, part, prev);
+    launch_mode<T, 1>(c, mode, hv, xcur, xnext, ldc, j0, nvalid, p, part, prev, r);
+}
+
+// Light cones of the unit-vector chunks.  t_{n+1} = T_{n+1}(H~) e_j is zero
+// on every atom more than n + 1 hops (in H~'s block graph) from atom j/4, so
+// a step of the chunk j0 .. j0 + ld - 1 only has to visit the atoms within
+// n + 1 hops of its own.  Per chunk and hop count h: the cyclic atom range
+// {lo, len} around the chunk's centre that holds every atom within h hops
+// (len = nat once it holds them all).  The ranges only grow with h, which is
+// what keeps the rows a restricted step leaves alone at zero: steps 0 and 1
+// write every row, and a row outside the range of step n was outside the
+// smaller range of step n - 2 as well, so the buffer it is read from (or
+// written over) still holds that step's exact zero.
+int build_cones(ltb_ctx* c, int ld) {
+  if (c->cone_ld == ld) return LTB_OK;
+  const int nat = c->nat;
+  if ((int)c->h_rowptr.size() != nat + 1) {
+    c->h_rowptr.resize(nat + 1);
+    c->h_col.resize(c->nnzb);
+    LTB_HIP(c, hipMemcpy(c->h_rowptr.data(), c->d_rowptr, sizeof(int) * (nat + 1), hipMemcpyDeviceToHost));
+    LTB_HIP(c, hipMemcpy(c->h_col.data(), c->d_col, sizeof(int) * c->nnzb, hipMemcpyDeviceToHost));
+  }
+  const int apc = ld / 4;  // atoms per chunk
+  std::vector<char> seen(nat);
+  std::vector<int> frontier, next;
+  c->cones.clear();
+  for (int a0 = 0; a0 < nat; a0 += apc) {
+    const int a1 = std::min(a0 + apc, nat), centre = (a0 + a1) / 2;
+    // offset of an atom from the centre in [-nat/2, nat - nat/2)
+    auto off = [&](int a) { return (a - centre + nat + nat / 2) % nat - nat / 2; };
+    std::fill(seen.begin(), seen.end(), 0);
+    frontier.clear();
+    int lo = off(a0), hi = off(a1 - 1);
+    for (int a = a0; a < a1; a++) {
+      seen[a] = 1;
+      frontier.push_back(a);
+    }
+    std::vector<std::pair<int, int>> ranges;
+    auto push = [&]() {
+      const int len = hi - lo + 1;
+      ranges.push_back(len >= nat ? std::make_pair(0, nat) : std::make_pair(((centre + lo) % nat + nat) % nat, len));
+    };
+    push();
+    while (!frontier.empty() && ranges.back().second < nat) {
+      next.clear();
+      for (int a : frontier)
+        for (int k = c->h_rowptr[a]; k < c->h_rowptr[a + 1]; k++) {
+          const int b = c->h_col[k];
+          if (!seen[b]) {
+            seen[b] = 1;
+            next.push_back(b);
+            lo = std::min(lo, off(b));
+            hi = std::max(hi, off(b));
+          }
+        }
+      frontier.swap(next);
+      push();
+    }
+    c->cones.push_back(std::move(ranges));
+  }
+  c->cone_ld = ld;
+  return LTB_OK;
 }
 
 template <typename T>
@@ -357,6 +432,9 @@ int run_moments(ltb_ctx* c, int M) {
     }
   }
   const double hbytes = (double)c->nnzb * (16.0 * sizeof(T) + 4.0) + 4.0 * (c->nat + 1);
+  // light cones only matter with more than one chunk, on the natural order
+  const bool cones = c->light_cone && p.schedule == LTB_SCHED_SWEEP && p.order == 2 && c->norb > p.ld;
+  if (cones && (st = build_cones(c, p.ld))) return st;
   int launches = 0;
   double bytes = 0;
   for (int j0 = 0; j0 < c->norb; j0 += p.ld) {
@@ -390,12 +468,28 @@ int run_moments(ltb_ctx* c, int M) {
       T* nxt = (n == 0) ? B : ((n & 1) ? A : B);
       // under the sweep schedule step n folds step n - 1's partials itself
       const FoldArgs prev = sweep && n > 0 ? fold_args(n - 1) : FoldArgs{};
+      // steps 0 and 1 write every row; later sweep steps visit the cone of t_{n+1}
+      AtomRange r;
+      if (cones && sweep && n >= 2) {
+        const auto& cr = c->cones[j0 / p.ld];
+        const auto& e = cr[std::min<size_t>(n + 1, cr.size() - 1)];
+        r.lo = e.first;
+        r.len = e.second;
+      }
       if (c->timing) LTB_HIP(c, hipEventRecord(c->events[2 * launches], c->stream));
-      launch_step<T>(c, mode, hv, cur, nxt, ldc, j0, nvalid, pc, c->d_part + (n & 1) * half, prev);
+      launch_step<T>(c, mode, hv, cur, nxt, ldc, j0, nvalid, pc, c->d_part + (n & 1) * half, prev, r);
       if (c->timing) LTB_HIP(c, hipEventRecord(c->events[2 * launches + 1], c->stream));
       LTB_HIP(c, hipGetLastError());
       const int streams = mode == 0 ? 1 : (mode == 1 ? 2 : 3);
-      bytes += (double)streams * sizeof(T) * c->norb * (double)nvalid + hbytes;
+      if (r.len < 0 || r.len >= c->nat) {
+        bytes += (double)streams * sizeof(T) * c->norb * (double)nvalid + hbytes;
+      } else {  // the rows and blocks of the visited atoms only
+        const int* rp = c->h_rowptr.data();
+        const int e = r.lo + r.len;
+        const double nb = e <= c->nat ? rp[e] - rp[r.lo] : (rp[c->nat] - rp[r.lo]) + rp[e - c->nat];
+        bytes += (double)streams * sizeof(T) * 4.0 * r.len * (double)nvalid + nb * (16.0 * sizeof(T) + 4.0) +
+                 4.0 * (r.len + 1);
+      }
       launches++;
       if (!sweep || n == nsteps - 1) {
         hipLaunchKernelGGL(k_cheb_reduce, dim3(ldc / 64), dim3(256), 0, c->stream, fold_args(n));

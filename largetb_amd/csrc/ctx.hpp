@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/ltb.h"
@@ -78,6 +79,14 @@ struct ltb_ctx {
   // device address of the allocation-free entry point's scratch array on this
   // context's device (not owned: the array lives as long as the process)
   void* d_scratch = nullptr;
+  // light cones of the unit-vector chunks (sweep schedule): for chunk k of
+  // cone_ld columns and hop count h, the cyclic atom range {lo, len} holding
+  // every atom within h hops of the chunk's atoms; built from a host copy of
+  // the block pattern, dropped with the system
+  bool light_cone = true;
+  int cone_ld = -1;
+  std::vector<std::vector<std::pair<int, int>>> cones;
+  std::vector<int> h_rowptr, h_col;
 };
 
 int set_err(ltb_ctx* c, int code, const std::string& msg);

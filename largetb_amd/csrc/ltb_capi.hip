@@ -626,6 +626,10 @@ void release_system(ltb_ctx* c) {
   c->mom_M = -1;
   c->nat = c->norb = c->nnzb = c->gnnz = 0;
   c->nelectrons = 0;
+  c->cone_ld = -1;
+  c->cones.clear();
+  c->h_rowptr.clear();
+  c->h_col.clear();
 }
 
 // Allocate the buffers of a system with nat atoms, nnzb H blocks and gnnz
@@ -1084,6 +1088,12 @@ int ltb_scc_run(ltb_ctx* c, const ltb_scc_params* prm, const double* guess, doub
 int ltb_set_kpe_orbital_moments(ltb_ctx* c, int on) {
   if (!c) return LTB_ERR_ARG;
   c->keep_orbital = on != 0;
+  return LTB_OK;
+}
+
+int ltb_set_kpe_light_cone(ltb_ctx* c, int on) {
+  if (!c) return LTB_ERR_ARG;
+  c->light_cone = on != 0;
   return LTB_OK;
 }
 

@@ -113,6 +113,52 @@ def test_every_tiling_gives_the_same_moments(ctx, prec):
         ctx.set_tiling()
 
 
+def quartz_cluster(cells, sigma=0.05, seed=1):
+    """the periodic cell's atoms as a finite cluster (no images)"""
+    z, xyz = L.System.quartz(*cells, sigma_ang=sigma, seed=seed).coords()
+    return L.System.from_coords(z, xyz, None, periodic_mask=0)
+
+
+CONES = [  # system, tiling (sweep schedule, atom-index order)
+    (lambda: L.System.quartz(8, 8, 8, sigma_ang=0.05, seed=1), dict(chunk_cols=4096, cols_per_lane=4)),
+    (lambda: L.System.quartz(8, 8, 8, sigma_ang=0.05, seed=1), dict(chunk_cols=1024, cols_per_lane=1)),
+    # 1728 orbitals in chunks of 1024: the first chunk holds more than half the atoms
+    (lambda: L.System.quartz(4, 4, 3, sigma_ang=0.1, seed=3), dict(chunk_cols=1024, cols_per_lane=1)),
+    (lambda: quartz_cluster((3, 3, 3)), dict(chunk_cols=512, cols_per_lane=1)),
+]
+
+
+@pytest.mark.parametrize("prec", [L.FP64, L.FP32_ACC64])
+@pytest.mark.parametrize("case", range(len(CONES)))
+def test_light_cone_equals_the_full_sweep(ctx, case, prec):
+    """Sweep steps restricted to each chunk's light cone give the full sweep's
+    moments (the vectors are the same; only the order of the FP64 dot sums
+    changes) for low orders, where the cones are narrow, and for M = 512,
+    where they fill the cell; the counted bytes shrink accordingly."""
+    make, tiling = CONES[case]
+    s = make()
+    prepare(ctx, s)
+    try:
+        ctx.set_tiling(schedule=L.SCHED_SWEEP, blocks_per_cu=2, nontemporal=1, order=2, **tiling)
+        for M in (32, 512):
+            ctx.set_timing(True)
+            ctx.set_light_cone(False)
+            full, tfull = ctx.kpe_moments(M, prec)
+            bfull = ctx.kpe_stats()["bytes"]
+            ctx.set_light_cone(True)
+            cone, tcone = ctx.kpe_moments(M, prec)
+            bcone = ctx.kpe_stats()["bytes"]
+            assert np.abs(cone - full).max() < 1e-12 * max(1.0, np.abs(full).max()), M
+            assert np.abs(tcone - tfull).max() < 1e-12 * np.abs(tfull).max(), M
+            assert bcone <= bfull
+            if M == 32 and s.nat > 1000:
+                assert bcone < 0.95 * bfull
+    finally:
+        ctx.set_light_cone(True)
+        ctx.set_timing(False)
+        ctx.set_tiling()
+
+
 def test_autotune_picks_a_valid_tiling(ctx):
     s = L.System.quartz(8, 8, 8, sigma_ang=0.05, seed=1)
     prepare(ctx, s)
