@@ -85,6 +85,25 @@ def test_two_level_mixed_against_plain_fp64_at_4608_atoms(ctx):
     assert r["q"].sum() == pytest.approx(s.nelectrons, abs=1e-6)
 
 
+def test_two_level_cap_and_converged_start(ctx, c0):
+    """The iteration cap counts both kinds of iteration and reports
+    NOT_CONVERGED; started from its own answer, the two-level SCC stops after
+    its first (full-order) iteration with the same charges; FP32 throughout
+    (switch_tol 0) converges on an FP32 full-order iteration within the FP32
+    rounding of the FP64 answer."""
+    s, _ = c0
+    ctx.load(s)
+    r = ctx.scc(200, precision=L.FP64, tol=1e-6, max_iter=7, M_lo=32, check=False)
+    assert r["status"] == L.LTB_NOT_CONVERGED and r["iters"] == 7 and r["lo_iters"] == 6
+    ref = ctx.scc(200, precision=L.FP64, tol=1e-6, max_iter=1000, M_lo=32)
+    again = ctx.scc(200, guess=ref["pop"], precision=L.FP64, tol=1e-6, max_iter=1000, M_lo=32)
+    assert again["status"] == 0 and again["iters"] == 1 and again["lo_iters"] == 0
+    assert np.abs(again["q"] - ref["q"]).max() < 1e-7
+    f32 = ctx.scc(200, precision=L.FP32_ACC64, switch_tol=0.0, tol=1e-6, max_iter=1000, M_lo=32, use_graph=1)
+    assert f32["status"] == 0 and f32["fp64_iters"] == 0
+    assert np.abs(f32["q"] - ref["q"]).max() < 2e-6
+
+
 def test_two_level_parameter_checks(ctx, c0):
     s, _ = c0
     ctx.load(s)
