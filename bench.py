@@ -8,7 +8,8 @@ everything else FP64), inputs resident in HBM.
 
 value = algorithmic Chebyshev block-SpMM bytes of the timed iterations divided
 by their wall time (GB/s).  roofline = the block-SpMM kernel's algorithmic
-bytes per launch over its mean launch time (HIP events on the context stream).
+bytes per launch over its mean launch time (HIP events on the context stream
+around every launch of the first timed iteration).
 time_to_solution (default on, --no-tts): plain FP64 SCC from the neutral
 start against the learned guess with the two-level SCC (low-order iterations
 at M=32 corrected by the defect of the last full-order iteration, FP32
@@ -817,7 +818,8 @@ def main():
         log(f"warmup {args.warmup} iteration(s): {r['seconds']:.3f} s")
     ctx.set_timing(True)
     # every C-ABI call returns after its stream has drained, so the wall clock
-    # around ltb_scc_run brackets exactly the timed device work
+    # around ltb_scc_run brackets exactly the timed device work; HIP events
+    # around every step launch of its first iteration give the kernel time
     t0 = time.perf_counter()
     r = ctx.scc(args.M, precision=prec, max_iter=args.steps, tol=1e-30, check=False)
     wall = time.perf_counter() - t0

@@ -363,13 +363,17 @@ typedef struct {
 int ltb_sp2_dense(ltb_ctx* ctx, double Ne, int max_iter, int fp64_tail, double* diagP, double* pop,
                   ltb_sp2_result* res);
 
-/* Timing of the last ltb_kpe_moments call: number of recursion-step launches,
- * their summed device time (HIP events on the context stream), and the
- * algorithmic bytes they move (3 vector streams + H~ blocks + row pointers). */
+/* Timing of the last exact-trace moment computation made while timing was on
+ * (an ltb_kpe_moments call, or the first iteration of an ltb_scc_run):
+ * number of recursion-step launches, their summed device time (HIP events on
+ * the context stream), and the algorithmic bytes they move (3 vector streams
+ * + H~ blocks + row pointers, over the rows visited). */
 int ltb_kpe_stats(ltb_ctx* ctx, int* launches, double* step_seconds, double* alg_bytes);
 
 /* Record HIP events around every recursion-step launch (on the context
- * stream) so ltb_kpe_stats reports their summed device time. */
+ * stream) so ltb_kpe_stats reports their summed device time.  ltb_scc_run
+ * records them in its first iteration only (events around every launch
+ * lengthen an iteration by ~1.4%). */
 int ltb_set_kpe_timing(ltb_ctx* ctx, int on);
 
 /* Work mapping of the Chebyshev block-SpMM kernel.  No reference counterpart

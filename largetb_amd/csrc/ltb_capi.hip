@@ -1041,6 +1041,7 @@ int ltb_scc_run(ltb_ctx* c, const ltb_scc_params* prm, const double* guess, doub
       LTB_HIP(c, hipGraphLaunch(exec[gi], c->stream));
       ngraph++;
     } else {
+      c->timing = saved_timing && it == 0;  // per-launch events in the first iteration only
       if ((s2 = scc_iteration_async(c, prm, &fp, prec))) return s2;
       warmed[gi] = true;
     }
