@@ -826,6 +826,7 @@ def main():
     st = ctx.kpe_stats()
     log(f"timed {args.steps} iteration(s): {wall:.3f} s, dq={r['dq']:.3e}")
     alg_bytes_iter = st["bytes"]
+    traffic = measured_traffic(args.cells, args.M, tiling, args.precision)
     value = alg_bytes_iter * args.steps / wall / 1e9
     achieved = st["bytes"] / st["seconds"] / 1e9
     out = {
@@ -842,12 +843,14 @@ def main():
                    "seed": args.seed},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(args.cells, args.M, tiling, args.precision),
+                     "traffic": traffic,
                      "kernel": "k_cheb_sweep" if tiling and tiling.get("schedule") == 2 else "k_cheb_blocked",
                      "launches": st["launches"],
                      "mean_launch_us": round(st["seconds"] / max(st["launches"], 1) * 1e6, 2),
                      "tiling": tiling},
     }
+    if traffic is not None:  # the launch the counters measured
+        out["roofline"]["traffic_launch"] = "full-width general step (every atom visited), 3.062 GB algorithmic"
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cells, args.sigma, args.seed, args.M, alg_bytes_iter)
         out["cpu_baseline"]["c0_full_scc"] = c0_scc_both(ctx)
