@@ -128,6 +128,12 @@ int ltb_set_populations(ltb_ctx* ctx, const double* p);
  * Si 2, O 3); the reference's literal values (Si 2, O 0: :714-720) for
  * comparisons with its runs */
 int ltb_set_neutral(ltb_ctx* ctx, const double* p0);
+/* replace Gamma_AA (per atom) of the loaded system.  The reference sets
+ * Gamma_AA = -Us with Us = 1.0 for every atom (initializeForSCF :876, :887), the
+ * default here; its Slater-Koster routine declares Hubbard parameters it never
+ * uses, Ud = 0.247609 for Si (:261) and Up = 0.4954 for O (:303), which with
+ * Gamma_AA = +U give the on-site Coulomb term of standard SCC tight binding. */
+int ltb_set_gamma_onsite(ltb_ctx* ctx, const double* gamma_aa);
 int ltb_get_shift(ltb_ctx* ctx, double* v_atom);
 
 /* GershGorinParallel (:144-209) on H_total, FP64 device reduction. */

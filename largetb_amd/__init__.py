@@ -63,6 +63,16 @@ class KpeTiling(C.Structure):
 SCHED_AUTO, SCHED_BLOCKED, SCHED_SWEEP = 0, 1, 2
 
 
+# Hubbard parameters the reference declares in its on-site Slater-Koster
+# blocks but never uses (withSP2_deltaH.cpp:261 Si Ud, :303 O Up), Hartree
+HUBBARD_U = {14: 0.247609, 8: 0.4954}
+
+
+def hubbard_u(Z):
+    """Gamma_AA = +U per atom for ltb_set_gamma_onsite (the +U variant)."""
+    return np.array([HUBBARD_U[int(z)] for z in Z], np.float64)
+
+
 class SccParams(C.Structure):
     _fields_ = [("M", C.c_int), ("kT", C.c_double), ("K", C.c_int), ("mix", C.c_double),
                 ("tol", C.c_double), ("max_iter", C.c_int), ("precision", C.c_int),
@@ -133,6 +143,7 @@ SIGNATURES = {
     "ltb_kpe_stats": (C.c_int, [VP, PI, PD, PD]),
     "ltb_set_kpe_orbital_moments": (C.c_int, [VP, C.c_int]),
     "ltb_set_kpe_light_cone": (C.c_int, [VP, C.c_int]),
+    "ltb_set_gamma_onsite": (C.c_int, [VP, PD]),
     "ltb_density_diag": (C.c_int, [VP, PD]),
     "ltb_get_orbital_moments": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, PD]),
     "ltb_kpe_weights": (C.c_int, [VP, C.c_int, PD]),
@@ -526,6 +537,10 @@ class Context:
 
     def keep_orbital_moments(self, on=True):
         self._check(lib().ltb_set_kpe_orbital_moments(self._h, 1 if on else 0))
+
+    def set_gamma_onsite(self, gamma_aa):
+        """Gamma_AA per atom (ltb_set_gamma_onsite); hubbard_u(Z) gives the +U values."""
+        self._check(lib().ltb_set_gamma_onsite(self._h, _d(_vec(gamma_aa, self.nat, "gamma_aa"))))
 
     def set_light_cone(self, on=True):
         """Light-cone restriction of the unit-vector recursion (default on)."""

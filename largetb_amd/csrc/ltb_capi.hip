@@ -311,6 +311,15 @@ __global__ void k_mix(double* __restrict__ pin, const double* __restrict__ pout,
   if (a < nat) pin[a] += mix * (pout[a] - pin[a]);
 }
 
+// Gamma_AA of every atom (the diagonal entry of its CSR row)
+__global__ void k_gamma_onsite(const int* __restrict__ grow, const int* __restrict__ gcol,
+                               double* __restrict__ gval, const double* __restrict__ u, int nat) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= nat) return;
+  for (int k = grow[a]; k < grow[a + 1]; k++)
+    if (gcol[k] == a) gval[k] = u[a];
+}
+
 __global__ void k_clear_flag(DevScalars* sc) {
   if (threadIdx.x == 0) sc->nonfinite = 0;
 }
@@ -724,6 +733,26 @@ int ltb_get_shift(ltb_ctx* c, double* v) {
   if (!v) return set_err(c, LTB_ERR_ARG, "null output");
   LTB_HIP(c, hipMemcpyAsync(v, c->d_v, sizeof(double) * c->nat, hipMemcpyDeviceToHost, c->stream));
   LTB_HIP(c, hipStreamSynchronize(c->stream));
+  return LTB_OK;
+}
+
+int ltb_set_gamma_onsite(ltb_ctx* c, const double* gamma_aa) {
+  int st = need_system(c);
+  if (st) return st;
+  if (!gamma_aa) return set_err(c, LTB_ERR_ARG, "null onsite Gamma");
+  for (int i = 0; i < c->nat; i++)
+    if (!std::isfinite(gamma_aa[i])) return set_err(c, LTB_ERR_NONFINITE, "non-finite onsite Gamma");
+  double* d = nullptr;
+  LTB_HIP(c, hipMalloc(&d, sizeof(double) * c->nat));
+  hipError_t e = hipMemcpy(d, gamma_aa, sizeof(double) * c->nat, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_gamma_onsite, dim3(grid_for(c->nat, 256)), dim3(256), 0, c->stream, c->d_grow, c->d_gcol,
+                       c->d_gval, d, c->nat);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  }
+  (void)hipFree(d);
+  if (e != hipSuccess) return set_err(c, LTB_ERR_DEVICE, std::string("onsite Gamma: ") + hipGetErrorString(e));
   return LTB_OK;
 }
 

@@ -174,6 +174,13 @@ class System:
             H[np.diag_indices(n)] += shift
         return H
 
+    def set_gamma_onsite(self, gamma_aa):
+        """Gamma_AA per atom (the diagonal entries of the CSR rows), in place."""
+        for i in range(self.nat):
+            for k in range(self.g_rowptr[i], self.g_rowptr[i + 1]):
+                if self.g_col[k] == i:
+                    self.g_val[k] = gamma_aa[i]
+
     def dense_gamma(self):
         G = np.zeros((self.nat, self.nat))
         for i in range(self.nat):

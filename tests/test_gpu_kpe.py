@@ -257,6 +257,31 @@ def test_scc_mixed_precision_matches_fp64(ctx, c0, c0_fixed_point):
     assert abs(ref["eband"] - estar) * EV / s.nat > 1e-5
 
 
+def test_scc_with_hubbard_onsite_gamma_matches_oracle(ctx):
+    """The +U variant (Gamma_AA = U_Si, U_O from the reference's unused
+    Hubbard parameters) through ltb_set_gamma_onsite: the device Gamma holds U
+    on its diagonal, the FP64 SCC equals the oracle's with the same Gamma
+    (1e-8 e), and reloading the system restores the reference's -1."""
+    s, o = pair((2, 2, 2))
+    ctx.load(s)
+    z, _ = s.coords()
+    u = L.hubbard_u(z)
+    ctx.set_gamma_onsite(u)
+    rp, col, val = ctx.gamma()
+    diag = np.array([val[k] for i in range(s.nat) for k in range(rp[i], rp[i + 1]) if col[k] == i])
+    assert np.array_equal(diag, u)
+    r = ctx.scc(200, precision=L.FP64, tol=1e-6, max_iter=600)
+    o.set_gamma_onsite(u)
+    ref = o.scc(200, tol=1e-6, maxit=600)
+    assert ref["status"] == 0 and abs(r["iters"] - ref["iters"]) <= 1
+    assert np.abs(r["q"] - ref["q"]).max() < 1e-8
+    ctx.load(s)
+    rp, col, val = ctx.gamma()
+    assert all(val[k] == -1.0 for i in range(s.nat) for k in range(rp[i], rp[i + 1]) if col[k] == i)
+    with pytest.raises(ValueError):
+        ctx.set_gamma_onsite(u[:-1])
+
+
 def test_scc_graph_replays_match_eager(ctx, c0):
     """hipGraph-captured iterations (replayed many times, host reads between
     replays) reproduce the eager loop bit for bit."""

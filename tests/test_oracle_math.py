@@ -179,6 +179,29 @@ def test_scc_fixed_point_and_guess_independence(c0):
     assert r3["status"] == 5  # ORC_NOT_CONVERGED
 
 
+def test_scc_with_hubbard_onsite_gamma():
+    """The +U variant (Gamma_AA = U_Si 0.247609, U_O 0.4954 Ha in place of the
+    reference's -1): Gamma's diagonal holds U, the SCC converges to a fixed
+    point q = F(q) of that map (a fresh density step reproduces it), from the
+    neutral start and from a perturbed one, and it is not the -1 variant's."""
+    o = O.System.quartz(2, 2, 2, sigma_ang=0.05, seed=1)
+    base = o.scc(200, tol=1e-8, maxit=600)
+    u = np.where(o.Z == 14, 0.247609, 0.4954)
+    o.set_gamma_onsite(u)
+    assert np.array_equal(np.diag(o.dense_gamma()), u)
+    r = o.scc(200, tol=1e-8, maxit=600)
+    assert r["status"] == 0 and 40 <= r["iters"] <= 120
+    v = o.dense_gamma() @ (r["pop"] - o.p0)
+    d = o.density(200, shift=np.repeat(v, 4))
+    assert np.abs(2 * d["pop"] - r["q"]).max() < 2e-9
+    rng = np.random.default_rng(4)
+    g = r["pop"] + 1e-3 * rng.standard_normal(o.nat)
+    g -= (g.sum() - r["pop"].sum()) / o.nat
+    r2 = o.scc(200, guess=g, tol=1e-8, maxit=600)
+    assert np.abs(r2["q"] - r["q"]).max() < 1e-7
+    assert np.abs(r["q"] - base["q"]).max() > 1e-3
+
+
 def test_two_level_scc_reaches_the_plain_fixed_point(c0):
     """The two-level SCC (low-order iterations corrected by the defect of the
     last full-order one) stops at the plain SCC's fixed point: the final
