@@ -342,18 +342,19 @@ void launch_step(const ltb_ctx* c, int mode, const T* hv, const T* xcur, T* xnex
 // smaller range of step n - 2 as well, so the buffer it is read from (or
 // written over) still holds that step's exact zero.
 int build_cones(ltb_ctx* c, int ld) {
-  if (c->cone_ld == ld) return LTB_OK;
+  if (c->cones.count(ld)) return LTB_OK;
   const int nat = c->nat;
-  if ((int)c->h_rowptr.size() != nat + 1) {
-    c->h_rowptr.resize(nat + 1);
-    c->h_col.resize(c->nnzb);
-    LTB_HIP(c, hipMemcpy(c->h_rowptr.data(), c->d_rowptr, sizeof(int) * (nat + 1), hipMemcpyDeviceToHost));
-    LTB_HIP(c, hipMemcpy(c->h_col.data(), c->d_col, sizeof(int) * c->nnzb, hipMemcpyDeviceToHost));
+  if ((int)c->h_rowptr.size() != nat + 1) {  // kept only once both copies succeeded
+    std::vector<int> rp(nat + 1), cl(c->nnzb);
+    LTB_HIP(c, hipMemcpy(rp.data(), c->d_rowptr, sizeof(int) * (nat + 1), hipMemcpyDeviceToHost));
+    LTB_HIP(c, hipMemcpy(cl.data(), c->d_col, sizeof(int) * c->nnzb, hipMemcpyDeviceToHost));
+    c->h_rowptr.swap(rp);
+    c->h_col.swap(cl);
   }
   const int apc = ld / 4;  // atoms per chunk
   std::vector<char> seen(nat);
   std::vector<int> frontier, next;
-  c->cones.clear();
+  auto& cones = c->cones[ld];
   for (int a0 = 0; a0 < nat; a0 += apc) {
     const int a1 = std::min(a0 + apc, nat), centre = (a0 + a1) / 2;
     // offset of an atom from the centre in [-nat/2, nat - nat/2)
@@ -386,9 +387,8 @@ int build_cones(ltb_ctx* c, int ld) {
       frontier.swap(next);
       push();
     }
-    c->cones.push_back(std::move(ranges));
+    cones.push_back(std::move(ranges));
   }
-  c->cone_ld = ld;
   return LTB_OK;
 }
 
@@ -471,7 +471,7 @@ int run_moments(ltb_ctx* c, int M) {
       // steps 0 and 1 write every row; later sweep steps visit the cone of t_{n+1}
       AtomRange r;
       if (cones && sweep && n >= 2) {
-        const auto& cr = c->cones[j0 / p.ld];
+        const auto& cr = c->cones[p.ld][j0 / p.ld];
         const auto& e = cr[std::min<size_t>(n + 1, cr.size() - 1)];
         r.lo = e.first;
         r.len = e.second;

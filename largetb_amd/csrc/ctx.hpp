@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <string>
 #include <utility>
 #include <vector>
@@ -79,13 +80,12 @@ struct ltb_ctx {
   // device address of the allocation-free entry point's scratch array on this
   // context's device (not owned: the array lives as long as the process)
   void* d_scratch = nullptr;
-  // light cones of the unit-vector chunks (sweep schedule): for chunk k of
-  // cone_ld columns and hop count h, the cyclic atom range {lo, len} holding
+  // light cones of the unit-vector chunks (sweep schedule), per chunk width
+  // ld: for chunk k and hop count h, the cyclic atom range {lo, len} holding
   // every atom within h hops of the chunk's atoms; built from a host copy of
   // the block pattern, dropped with the system
   bool light_cone = true;
-  int cone_ld = -1;
-  std::vector<std::vector<std::pair<int, int>>> cones;
+  std::map<int, std::vector<std::vector<std::pair<int, int>>>> cones;
   std::vector<int> h_rowptr, h_col;
 };
 

@@ -635,7 +635,6 @@ void release_system(ltb_ctx* c) {
   c->mom_M = -1;
   c->nat = c->norb = c->nnzb = c->gnnz = 0;
   c->nelectrons = 0;
-  c->cone_ld = -1;
   c->cones.clear();
   c->h_rowptr.clear();
   c->h_col.clear();
