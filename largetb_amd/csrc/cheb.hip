@@ -145,8 +145,8 @@ __global__ __launch_bounds__(256) void k_cheb_sweep(const int* __restrict__ rowp
                                                     T* __restrict__ xnext, int nat, int ld, int j0,
                                                     int nvalid, int n_cg, int slots,
                                                     const int* __restrict__ visit,
-                                                    double* __restrict__ part, FoldArgs prev, int alo,
-                                                    int alen) {
+                                                    double* __restrict__ part, FoldArgs prev,
+                                                    const int* __restrict__ ranges, int nranges) {
   constexpr int W = 64 * V;
   // the previous step's partials (complete: it was the previous launch) are
   // folded here instead of in a launch of their own
@@ -162,12 +162,25 @@ __global__ __launch_bounds__(256) void k_cheb_sweep(const int* __restrict__ rowp
     double sxy[V], syy[V];
 #pragma unroll
     for (int v = 0; v < V; v++) sxy[v] = syy[v] = 0.0;
-    // the atoms alo .. alo + alen - 1 (cyclic): the chunk's light cone, or all
-    for (int i = wid; i < alen; i += S) {
-      int A = alo + i;
-      A -= A >= nat ? nat : 0;
-      atom_step<T, V, MODE, NT>(rowptr, col, hv, xcur, xnext, visit ? visit[A] : A, ld, j0, c0, nvalid, sxy,
-                                syy);
+    if (!ranges) {
+      for (int i = wid; i < nat; i += S)
+        atom_step<T, V, MODE, NT>(rowptr, col, hv, xcur, xnext, visit ? visit[i] : i, ld, j0, c0, nvalid, sxy,
+                                  syy);
+    } else {
+      // the chunk's light cone: cyclic atom ranges {lo, len}, swept as one
+      // sequence (wave w takes positions w, w + S, ... of their concatenation)
+      int base = 0;
+      for (int r = 0; r < nranges; r++) {
+        const int lo = ranges[2 * r], len = ranges[2 * r + 1];
+        int i = (wid - base) % S;
+        i += i < 0 ? S : 0;
+        for (; i < len; i += S) {
+          int A = lo + i;
+          A -= A >= nat ? nat : 0;
+          atom_step<T, V, MODE, NT>(rowptr, col, hv, xcur, xnext, A, ld, j0, c0, nvalid, sxy, syy);
+        }
+        base += len;
+      }
     }
     flush_partials<V>(sxy, syy, wave, lane, part, (size_t)(xcd * slots + slot) * 2, ld, cg * W);
   }
@@ -229,10 +242,11 @@ __global__ __launch_bounds__(256) void k_block_reduce(const double* __restrict__
   if (2 * n + 2 <= M) momcol[(size_t)(2 * n + 2) * ld + c] = 2.0 * syy - r2;
 }
 
-// atoms a launch of the sweep schedule visits: alo .. alo + alen - 1
-// (cyclic); alen < 0: all of them
+// atoms a launch of the sweep schedule visits: n cyclic ranges {lo, len} at
+// d (device, 2 ints each); d == nullptr: all of them
 struct AtomRange {
-  int lo = 0, len = -1;
+  const int* d = nullptr;
+  int n = 0;
 };
 
 struct Plan {
@@ -285,11 +299,9 @@ void launch_one(const ltb_ctx* c, const T* hv, const T* xcur, T* xnext, int ldc,
                 const Plan& p, double* part, const FoldArgs& prev, AtomRange r) {
   const int n_cg = ldc / (64 * V);
   if (p.schedule == LTB_SCHED_SWEEP) {
-    const bool all = r.len < 0 || r.len >= c->nat;
     hipLaunchKernelGGL((k_cheb_sweep<T, V, MODE, NT>), dim3(8 * p.slots), dim3(256), 0, c->stream,
                        c->d_rowptr, c->d_col, hv, xcur, xnext, c->nat, ldc, j0, nvalid, n_cg,
-                       p.slots, p.order == 1 ? c->d_visit : nullptr, part, prev, all ? 0 : r.lo,
-                       all ? c->nat : r.len);
+                       p.slots, p.order == 1 ? c->d_visit : nullptr, part, prev, r.d, r.n);
   } else {
     hipLaunchKernelGGL((k_cheb_blocked<T, V, MODE, NT>), dim3(n_cg * p.n_sg), dim3(256), 0,
                        c->stream, c->d_rowptr, c->d_col, hv, xcur, xnext, c->nat, ldc, j0, nvalid,
@@ -333,14 +345,59 @@ void launch_step(const ltb_ctx* c, int mode, const T* hv, const T* xcur, T* xnex
 
 // Light cones of the unit-vector chunks.  t_{n+1} = T_{n+1}(H~) e_j is zero
 // on every atom more than n + 1 hops (in H~'s block graph) from atom j/4, so
-// a step of the chunk j0 .. j0 + ld - 1 only has to visit the atoms within
-// n + 1 hops of its own.  Per chunk and hop count h: the cyclic atom range
-// {lo, len} around the chunk's centre that holds every atom within h hops
-// (len = nat once it holds them all).  The ranges only grow with h, which is
-// what keeps the rows a restricted step leaves alone at zero: steps 0 and 1
-// write every row, and a row outside the range of step n was outside the
-// smaller range of step n - 2 as well, so the buffer it is read from (or
-// written over) still holds that step's exact zero.
+// step n >= 2 of the chunk j0 .. j0 + ld - 1 only has to visit the atoms
+// within n + 1 hops of its own.  Steps 0 and 1 write every row.  A row a
+// later step leaves alone is then outside that step's cone and so outside
+// every earlier one: each value the buffers ever held there is an exact zero
+// (a product or sum of zeros), which is what its readers need.  Visiting a
+// superset of the cone is therefore always safe; per chunk and hop count h
+// the table holds the cone as at most kMaxRanges cyclic atom ranges {lo, len}
+// (runs of consecutive atoms, the closest runs merged), or {0, nat} once it
+// holds every atom.
+constexpr int kMaxRanges = 32;
+
+void free_cones_impl(ltb_ctx* c) {
+  for (auto& kv : c->cones)
+    if (kv.second.d_ranges) (void)hipFree(kv.second.d_ranges);
+  c->cones.clear();
+}
+
+// runs of `in` (cyclic), merged across the smallest gaps down to kMaxRanges
+void cone_ranges(const std::vector<char>& in, int nat, std::vector<std::pair<int, int>>& out) {
+  out.clear();
+  for (int a = 0; a < nat;) {
+    if (!in[a]) {
+      a++;
+      continue;
+    }
+    int b = a;
+    while (b < nat && in[b]) b++;
+    out.push_back({a, b - a});
+    a = b;
+  }
+  if (out.size() > 1 && out.front().first == 0 && out.back().first + out.back().second == nat) {
+    out.back().second += out.front().second;  // a run across the end of the atom order
+    out.erase(out.begin());
+  }
+  for (int gap = 8; (int)out.size() > kMaxRanges; gap *= 2) {
+    std::vector<std::pair<int, int>> m;
+    for (const auto& r : out) {
+      if (!m.empty()) {
+        const int end = m.back().first + m.back().second;  // may exceed nat only for the last run
+        if (r.first - end <= gap) {
+          m.back().second = r.first + r.second - m.back().first;
+          continue;
+        }
+      }
+      m.push_back(r);
+    }
+    out.swap(m);
+  }
+  int tot = 0;
+  for (const auto& r : out) tot += r.second;
+  if (tot >= nat) out.assign(1, {0, nat});
+}
+
 int build_cones(ltb_ctx* c, int ld) {
   if (c->cones.count(ld)) return LTB_OK;
   const int nat = c->nat;
@@ -352,43 +409,49 @@ int build_cones(ltb_ctx* c, int ld) {
     c->h_col.swap(cl);
   }
   const int apc = ld / 4;  // atoms per chunk
-  std::vector<char> seen(nat);
+  ltb_ctx::ConeTable t;
+  std::vector<char> in(nat);
   std::vector<int> frontier, next;
-  auto& cones = c->cones[ld];
+  std::vector<std::pair<int, int>> rs;
   for (int a0 = 0; a0 < nat; a0 += apc) {
-    const int a1 = std::min(a0 + apc, nat), centre = (a0 + a1) / 2;
-    // offset of an atom from the centre in [-nat/2, nat - nat/2)
-    auto off = [&](int a) { return (a - centre + nat + nat / 2) % nat - nat / 2; };
-    std::fill(seen.begin(), seen.end(), 0);
+    const int a1 = std::min(a0 + apc, nat);
+    std::fill(in.begin(), in.end(), 0);
     frontier.clear();
-    int lo = off(a0), hi = off(a1 - 1);
     for (int a = a0; a < a1; a++) {
-      seen[a] = 1;
+      in[a] = 1;
       frontier.push_back(a);
     }
-    std::vector<std::pair<int, int>> ranges;
-    auto push = [&]() {
-      const int len = hi - lo + 1;
-      ranges.push_back(len >= nat ? std::make_pair(0, nat) : std::make_pair(((centre + lo) % nat + nat) % nat, len));
-    };
-    push();
-    while (!frontier.empty() && ranges.back().second < nat) {
+    std::vector<std::pair<int, int>> idx;
+    for (;;) {
+      cone_ranges(in, nat, rs);
+      idx.push_back({(int)t.ranges.size(), (int)rs.size()});
+      t.ranges.insert(t.ranges.end(), rs.begin(), rs.end());
+      if (frontier.empty() || (rs.size() == 1 && rs[0].second >= nat)) break;
       next.clear();
       for (int a : frontier)
         for (int k = c->h_rowptr[a]; k < c->h_rowptr[a + 1]; k++) {
           const int b = c->h_col[k];
-          if (!seen[b]) {
-            seen[b] = 1;
+          if (!in[b]) {
+            in[b] = 1;
             next.push_back(b);
-            lo = std::min(lo, off(b));
-            hi = std::max(hi, off(b));
           }
         }
       frontier.swap(next);
-      push();
     }
-    cones.push_back(std::move(ranges));
+    t.idx.push_back(std::move(idx));
   }
+  std::vector<int> flat;
+  for (const auto& r : t.ranges) {
+    flat.push_back(r.first);
+    flat.push_back(r.second);
+  }
+  LTB_HIP(c, hipMalloc(&t.d_ranges, sizeof(int) * std::max<size_t>(flat.size(), 2)));
+  hipError_t e = hipMemcpy(t.d_ranges, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(t.d_ranges);
+    return set_err(c, LTB_ERR_DEVICE, std::string("light-cone upload: ") + hipGetErrorString(e));
+  }
+  c->cones[ld] = std::move(t);
   return LTB_OK;
 }
 
@@ -470,25 +533,32 @@ int run_moments(ltb_ctx* c, int M) {
       const FoldArgs prev = sweep && n > 0 ? fold_args(n - 1) : FoldArgs{};
       // steps 0 and 1 write every row; later sweep steps visit the cone of t_{n+1}
       AtomRange r;
+      const std::pair<int, int>* rr = nullptr;
       if (cones && sweep && n >= 2) {
-        const auto& cr = c->cones[p.ld][j0 / p.ld];
-        const auto& e = cr[std::min<size_t>(n + 1, cr.size() - 1)];
-        r.lo = e.first;
-        r.len = e.second;
+        const auto& t = c->cones[p.ld];
+        const auto& ci = t.idx[j0 / p.ld];
+        const auto& e = ci[std::min<size_t>(n + 1, ci.size() - 1)];
+        rr = t.ranges.data() + e.first;
+        if (!(e.second == 1 && rr->second >= c->nat)) {
+          r.d = t.d_ranges + 2 * (size_t)e.first;
+          r.n = e.second;
+        }
       }
       if (c->timing) LTB_HIP(c, hipEventRecord(c->events[2 * launches], c->stream));
       launch_step<T>(c, mode, hv, cur, nxt, ldc, j0, nvalid, pc, c->d_part + (n & 1) * half, prev, r);
       if (c->timing) LTB_HIP(c, hipEventRecord(c->events[2 * launches + 1], c->stream));
       LTB_HIP(c, hipGetLastError());
       const int streams = mode == 0 ? 1 : (mode == 1 ? 2 : 3);
-      if (r.len < 0 || r.len >= c->nat) {
+      if (!r.d) {
         bytes += (double)streams * sizeof(T) * c->norb * (double)nvalid + hbytes;
       } else {  // the rows and blocks of the visited atoms only
         const int* rp = c->h_rowptr.data();
-        const int e = r.lo + r.len;
-        const double nb = e <= c->nat ? rp[e] - rp[r.lo] : (rp[c->nat] - rp[r.lo]) + rp[e - c->nat];
-        bytes += (double)streams * sizeof(T) * 4.0 * r.len * (double)nvalid + nb * (16.0 * sizeof(T) + 4.0) +
-                 4.0 * (r.len + 1);
+        for (int q = 0; q < r.n; q++) {
+          const int lo = rr[q].first, len = rr[q].second, e = lo + len;
+          const double nb = e <= c->nat ? rp[e] - rp[lo] : (rp[c->nat] - rp[lo]) + rp[e - c->nat];
+          bytes += (double)streams * sizeof(T) * 4.0 * len * (double)nvalid + nb * (16.0 * sizeof(T) + 4.0) +
+                   4.0 * (len + 1);
+        }
       }
       launches++;
       if (!sweep || n == nsteps - 1) {
@@ -606,6 +676,8 @@ int kpe_moments_impl(ltb_ctx* c, int M, int precision) {
   if (precision == LTB_FP32_ACC64) return run_moments<float>(c, M);
   return run_moments<double>(c, M);
 }
+
+void free_cones(ltb_ctx* c) { free_cones_impl(c); }
 
 int kpe_reserve(ltb_ctx* c, int M, int precision) {
   if (precision == LTB_FP32_ACC64) return ensure_buffers<float>(c, make_plan<float>(c), M);

@@ -81,11 +81,16 @@ struct ltb_ctx {
   // context's device (not owned: the array lives as long as the process)
   void* d_scratch = nullptr;
   // light cones of the unit-vector chunks (sweep schedule), per chunk width
-  // ld: for chunk k and hop count h, the cyclic atom range {lo, len} holding
-  // every atom within h hops of the chunk's atoms; built from a host copy of
-  // the block pattern, dropped with the system
+  // ld: for chunk k and hop count h, a few cyclic atom ranges {lo, len} that
+  // hold every atom within h hops of the chunk's atoms; built from a host
+  // copy of the block pattern, dropped with the system
+  struct ConeTable {
+    std::vector<std::vector<std::pair<int, int>>> idx;  // [chunk][hop]: first range, number of ranges
+    std::vector<std::pair<int, int>> ranges;            // {lo, len} of every (chunk, hop), host copy
+    int* d_ranges = nullptr;                            // the same on the device, 2 ints per range
+  };
   bool light_cone = true;
-  std::map<int, std::vector<std::vector<std::pair<int, int>>>> cones;
+  std::map<int, ConeTable> cones;
   std::vector<int> h_rowptr, h_col;
 };
 
@@ -118,6 +123,8 @@ int ensure_buffer(ltb_ctx* c, void** p, size_t* cap, size_t bytes);
 int kpe_moments_impl(ltb_ctx* c, int M, int precision);
 // size the recursion, partial and moment buffers for order M in this precision
 int kpe_reserve(ltb_ctx* c, int M, int precision);
+// drop the light-cone tables (with the system or the context)
+void free_cones(ltb_ctx* c);
 // moments r_c^T T_m r_c of explicit vectors already in d_xa (row-major [norb][ld])
 int kpe_block_moments_impl(ltb_ctx* c, int M, int nvec, int precision, const double* d_r2, double* d_momcol,
                            int ld);

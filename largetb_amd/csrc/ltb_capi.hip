@@ -456,6 +456,7 @@ void free_all(ltb_ctx* c) {
                   c->lo_W, c->lo_X, c->lo_g, c->lo_w, c->d_plo, c->d_defect};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  free_cones(c);
 }
 
 }  // namespace
@@ -635,7 +636,7 @@ void release_system(ltb_ctx* c) {
   c->mom_M = -1;
   c->nat = c->norb = c->nnzb = c->gnnz = 0;
   c->nelectrons = 0;
-  c->cones.clear();
+  free_cones(c);
   c->h_rowptr.clear();
   c->h_col.clear();
 }
