@@ -17,6 +17,10 @@ mixing in place of updateHtotal).
   1e-6 e and the fixed-point band energy within 1e-6 eV/atom.
 * Batches (ltb_batch_scc_run with M_lo > 0): five uneven cells and the
   256-cell workload against each cell's own plain FP64 solve.
+* 15,552 atoms (12x12x12, sigma 0.10 A, M = 512) and 27,648 atoms (16x16x12,
+  sigma 0.08 A, M = 1024): the shipped learned guess plus the two-level SCC,
+  then one plain FP64 iteration from the answer (the plain solves take 5 and
+  33 minutes, too long to repeat here).
 """
 import numpy as np
 import pytest
@@ -178,16 +182,11 @@ def test_batch_two_level_of_256_cells(ctx):
         assert abs(r["eband_fp"][k] - g["eband_fp"]) * EV / systems[k].nat < 1e-6, k
 
 
-def test_c3_learned_two_level_reaches_the_plain_fixed_point(ctx):
-    """The headline workload (12x12x12, 15,552 atoms, sigma 0.10 A, seed 1,
-    M = 512) as the benchmark runs it: the shipped learned guess, then the
-    two-level SCC (M_lo = 32; FP32 full-order iterations until |dq| < 1e-2 e,
-    FP64 after) converges in at most 4 full-order iterations.  One plain FP64
-    iteration started from its answer moves every charge by < 1e-7 e and the
-    band energy by < 1e-7 eV/atom, so the answer is the fixed point the plain
-    FP64 solve converges to (that solve stops within ~1e-8 e of it)."""
+def learned_two_level_then_plain_iteration(ctx, s, M):
+    """The shipped learned guess, the two-level SCC as the benchmark runs it
+    (M_lo = 32; FP32 full-order iterations until |dq| < 1e-2 e, FP64 after),
+    then one plain FP64 iteration from its answer."""
     from largetb_amd import learned
-    s = L.System.quartz(12, 12, 12, sigma_ang=0.10, seed=1)
     ctx.load(s)
     ctx.set_populations(None)
     lo, hi = ctx.gershgorin()
@@ -197,14 +196,39 @@ def test_c3_learned_two_level_reaches_the_plain_fixed_point(ctx):
     m, _ = learned.load(ctx)
     guess = m.predict(s)
     m.close()
-    r = ctx.scc(512, guess=guess, precision=L.FP32_ACC64, switch_tol=1e-2, tol=1e-6, max_iter=2000, use_graph=1,
+    r = ctx.scc(M, guess=guess, precision=L.FP32_ACC64, switch_tol=1e-2, tol=1e-6, max_iter=2000, use_graph=1,
                 M_lo=32)
-    chk = ctx.scc(512, guess=r["pop"], precision=L.FP64, tol=1e-30, max_iter=1, check=False)
+    chk = ctx.scc(M, guess=r["pop"], precision=L.FP64, tol=1e-30, max_iter=1, check=False)
     full = r["iters"] - r["lo_iters"]
     deb = abs(chk["eband_fp"] - r["eband_fp"]) * EV / s.nat
-    print(f"15,552 atoms: learned guess off by {np.abs(2 * guess - r['q']).max():.2e} e; two-level {full} full-order "
-          f"({r['fp64_iters']} FP64) + {r['lo_iters']} low-order, {r['seconds']:.2f} s; plain FP64 iteration from the "
-          f"answer: |dq| {chk['hist'][0]:.2e} e, |dE_band| {deb:.2e} eV/atom")
+    print(f"{s.nat} atoms, M={M}: learned guess off by {np.abs(2 * guess - r['q']).max():.2e} e; two-level {full} "
+          f"full-order ({r['fp64_iters']} FP64) + {r['lo_iters']} low-order, {r['seconds']:.2f} s; plain FP64 "
+          f"iteration from the answer: |dq| {chk['hist'][0]:.2e} e, |dE_band| {deb:.2e} eV/atom")
+    return r, chk, full, deb
+
+
+def test_c3_learned_two_level_reaches_the_plain_fixed_point(ctx):
+    """The headline workload (12x12x12, 15,552 atoms, sigma 0.10 A, seed 1,
+    M = 512) as the benchmark runs it converges in at most 4 full-order
+    iterations.  One plain FP64 iteration started from its answer moves every
+    charge by < 1e-7 e and the band energy by < 1e-7 eV/atom, so the answer is
+    the fixed point the plain FP64 solve converges to (that solve stops within
+    ~1e-8 e of it)."""
+    s = L.System.quartz(12, 12, 12, sigma_ang=0.10, seed=1)
+    r, chk, full, deb = learned_two_level_then_plain_iteration(ctx, s, 512)
+    assert r["status"] == 0 and full <= 4 and r["fp64_iters"] >= 1
+    assert chk["hist"][0] < 1e-7 and deb < 1e-7
+    assert r["q"].sum() == pytest.approx(s.nelectrons, abs=1e-6)
+
+
+def test_c4_learned_two_level_reaches_the_plain_fixed_point(ctx):
+    """The largest single-GPU cell (16x16x12, 27,648 atoms, 110,592 orbitals,
+    sigma 0.08 A, seed 1) at its configured M = 1024, the full pipeline:
+    learned guess and two-level SCC in at most 4 full-order iterations, then
+    one plain FP64 iteration from the answer moves every charge by < 1e-7 e and
+    the band energy by < 1e-7 eV/atom."""
+    s = L.System.quartz(16, 16, 12, sigma_ang=0.08, seed=1)
+    r, chk, full, deb = learned_two_level_then_plain_iteration(ctx, s, 1024)
     assert r["status"] == 0 and full <= 4 and r["fp64_iters"] >= 1
     assert chk["hist"][0] < 1e-7 and deb < 1e-7
     assert r["q"].sum() == pytest.approx(s.nelectrons, abs=1e-6)
