@@ -398,6 +398,10 @@ typedef struct {
   int nontemporal;    /* 1: t_{n-1}, t_{n+1} bypass the caches; 0: cached; -1 auto */
   int order;          /* sweep visiting order of the atoms: 0 auto (= 2), 1 spatial
                          strips (bonded atoms close in sequence), 2 atom index   */
+  int lo_chunk_cols;  /* chunk width of the two-level SCC's low-order recursions
+                         (order M_lo), 0 = chunk_cols.  Under light cones a short
+                         recursion moves fewer bytes in narrower chunks; set by
+                         ltb_kpe_autotune.  Batches ignore it.                  */
 } ltb_kpe_tiling;
 /* The context keeps one tiling per recursion precision (FP32, FP64); this
  * sets both. */
@@ -407,7 +411,9 @@ int ltb_get_kpe_tiling(ltb_ctx* ctx, int precision, ltb_kpe_tiling* out);
 /* Time `trial_steps` recursion steps of every candidate tiling on the loaded
  * system, keep the fastest as the tiling of `precision` (the other
  * precision's tiling is left as it was) and report it with its algorithmic
- * GB/s. */
+ * GB/s.  With light cones on and a sweep over several chunks it then times a
+ * whole order-32 recursion at the chosen chunk width and at half of it and
+ * keeps the faster as lo_chunk_cols. */
 int ltb_kpe_autotune(ltb_ctx* ctx, int precision, int trial_steps, ltb_kpe_tiling* best,
                      double* best_gbs);
 

@@ -54,7 +54,7 @@ class DensityResult(C.Structure):
 class KpeTiling(C.Structure):
     _fields_ = [("chunk_cols", C.c_int), ("cols_per_lane", C.c_int), ("schedule", C.c_int),
                 ("blocks_per_cu", C.c_int), ("atoms_per_wave", C.c_int), ("nontemporal", C.c_int),
-                ("order", C.c_int)]
+                ("order", C.c_int), ("lo_chunk_cols", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -589,8 +589,9 @@ class Context:
         self._check(lib().ltb_set_kpe_timing(self._h, 1 if on else 0))
 
     def set_tiling(self, chunk_cols=0, cols_per_lane=0, schedule=SCHED_AUTO, blocks_per_cu=0,
-                   atoms_per_wave=0, nontemporal=-1, order=0):
-        t = KpeTiling(chunk_cols, cols_per_lane, schedule, blocks_per_cu, atoms_per_wave, nontemporal, order)
+                   atoms_per_wave=0, nontemporal=-1, order=0, lo_chunk_cols=0):
+        t = KpeTiling(chunk_cols, cols_per_lane, schedule, blocks_per_cu, atoms_per_wave, nontemporal, order,
+                      lo_chunk_cols)
         self._check(lib().ltb_set_kpe_tiling(self._h, C.byref(t)))
 
     def get_tiling(self, precision=FP32_ACC64):

@@ -495,7 +495,7 @@ struct ltb_batch {
   std::vector<void*> allocs;
   int *d_rowptr, *d_col, *d_blkrow, *d_atomst, *d_grow, *d_gcol, *d_aoff, *d_active;
   int nslots = 0;
-  ltb_kpe_tiling tiling{256, 4, LTB_SCHED_SWEEP, 8, 8, 1, 2};  // column group, V, -, blocks/CU, atoms/wave, NT, index order
+  ltb_kpe_tiling tiling{256, 4, LTB_SCHED_SWEEP, 8, 8, 1, 2, 0};  // column group, V, -, blocks/CU, atoms/wave, NT, index order, -
   void* d_tasks = nullptr;  // BatchTask per slot
   int *d_tbegin = nullptr, *d_nsg = nullptr;
   int *d_gc_s, *d_gc_c;

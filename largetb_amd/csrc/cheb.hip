@@ -689,8 +689,40 @@ int kpe_time_steps(ltb_ctx* c, int precision, int steps, double* seconds, double
   return time_steps<double>(c, steps, seconds, bytes);
 }
 
+int kpe_time_recursion(ltb_ctx* c, int precision, int M, int cols, int reps, double* seconds) {
+  KpeTiling& t = c->tiling[precision == LTB_FP64 ? 1 : 0];
+  const int saved_cols = t.chunk_cols;
+  const bool saved_timing = c->timing;
+  t.chunk_cols = cols;
+  c->timing = false;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  int st = e == hipSuccess ? LTB_OK : set_err(c, LTB_ERR_DEVICE, std::string("event: ") + hipGetErrorString(e));
+  double best = 0;
+  for (int r = -1; st == LTB_OK && r < reps; r++) {  // r = -1 warms up (and builds the light-cone table)
+    float ms = 0;
+    e = hipEventRecord(e0, c->stream);
+    if (e == hipSuccess && (st = kpe_moments_impl(c, M, precision)) != LTB_OK) break;
+    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) st = set_err(c, LTB_ERR_DEVICE, std::string("recursion timing: ") + hipGetErrorString(e));
+    if (r == 0 || (r > 0 && ms * 1e-3 < best)) best = ms * 1e-3;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  t.chunk_cols = saved_cols;
+  c->timing = saved_timing;
+  if (st == LTB_OK) *seconds = best;
+  return st;
+}
+
 int kpe_resolved_tiling(ltb_ctx* c, int precision, ltb_kpe_tiling* out) {
   Plan p = precision == LTB_FP32_ACC64 ? make_plan<float>(c) : make_plan<double>(c);
+  const int lo = c->tiling[precision == LTB_FP64 ? 1 : 0].lo_chunk_cols;
+  // the width make_plan gives the low-order recursions (their chunk_cols = lo)
+  out->lo_chunk_cols = lo > 0 ? (std::min(lo, c->norb) + p.W - 1) / p.W * p.W : p.ld;
   out->chunk_cols = p.ld;
   out->cols_per_lane = p.V;
   out->schedule = p.schedule;

@@ -68,7 +68,7 @@ struct ltb_ctx {
   double *d_plo = nullptr, *d_defect = nullptr;
   // tiling of the Chebyshev kernel per recursion precision, [0] FP32 and
   // [1] FP64 (0 / -1 fields = auto)
-  ltb_kpe_tiling tiling[2] = {{0, 0, 0, 0, 0, -1, 0}, {0, 0, 0, 0, 0, -1, 0}};
+  ltb_kpe_tiling tiling[2] = {{0, 0, 0, 0, 0, -1, 0, 0}, {0, 0, 0, 0, 0, -1, 0, 0}};
   // statistics of the last ltb_kpe_moments
   int stat_launches = 0;
   double stat_seconds = 0, stat_bytes = 0;
@@ -134,6 +134,9 @@ int kpe_block_width(ltb_ctx* c, int precision, int nvec);
 int kpe_apply_device(ltb_ctx* c, int M, const double* w, int ld, void* a, void* b, double* theta, int precision);
 int kpe_time_steps(ltb_ctx* c, int precision, int steps, double* seconds, double* bytes);
 int kpe_resolved_tiling(ltb_ctx* c, int precision, ltb_kpe_tiling* out);
+// whole exact-trace recursion of order M at chunk width `cols` (light cones as
+// set), device seconds of the fastest of `reps` runs after a warm-up
+int kpe_time_recursion(ltb_ctx* c, int precision, int M, int cols, int reps, double* seconds);
 typedef ltb_kpe_tiling KpeTiling;
 int noalloc_scratch_init(ltb_ctx* c);
 int sp2_dense_impl(ltb_ctx* c, double Ne, int max_iter, int fp64_tail, double* diagP, double* pop,

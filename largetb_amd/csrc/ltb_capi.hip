@@ -867,9 +867,22 @@ struct LowOrderFermi {
   }
 };
 
+// the low-order recursion's chunk width (tiling.lo_chunk_cols) while it runs
+struct LowOrderChunk {
+  ltb_kpe_tiling& t;
+  const int saved;
+  LowOrderChunk(ltb_ctx* c, int prec) : t(c->tiling[prec == LTB_FP64 ? 1 : 0]), saved(t.chunk_cols) {
+    if (t.lo_chunk_cols > 0) t.chunk_cols = t.lo_chunk_cols;
+  }
+  ~LowOrderChunk() { t.chunk_cols = saved; }
+  LowOrderChunk(const LowOrderChunk&) = delete;
+  LowOrderChunk& operator=(const LowOrderChunk&) = delete;
+};
+
 // populations at order fl->M of the H~ already rescaled at q_in, into d_pout
 int low_order_density_async(ltb_ctx* c, const ltb_fermi_params* fl, int prec) {
   LowOrderFermi sw(c);
+  LowOrderChunk ch(c, prec);
   int s2;
   if ((s2 = kpe_moments_impl(c, fl->M, prec))) return s2;
   return density_async(c, fl);
@@ -930,6 +943,10 @@ int scc_two_level(ltb_ctx* c, const ltb_scc_params* prm, const double* guess, do
   // buffers at their final size before anything is captured
   if ((st = kpe_reserve(c, prm->M, prm->precision))) return st;
   if (prm->switch_tol > 0 && (st = kpe_reserve(c, prm->M, LTB_FP64))) return st;
+  {
+    LowOrderChunk ch(c, prm->precision);
+    if ((st = kpe_reserve(c, prm->M_lo, prm->precision))) return st;
+  }
   const ltb_fermi_params fp{prm->M, prm->kT, prm->K, c->nelectrons};
   const ltb_fermi_params fl{prm->M_lo, prm->kT, prm->K, c->nelectrons};
   const double lo_tol = prm->lo_tol > 0 ? prm->lo_tol : 0.1 * prm->tol;
@@ -1200,7 +1217,7 @@ int ltb_set_kpe_timing(ltb_ctx* c, int on) {
 
 int ltb_set_kpe_tiling(ltb_ctx* c, const ltb_kpe_tiling* t) {
   if (!c || !t) return LTB_ERR_ARG;
-  if (t->chunk_cols < 0 || t->atoms_per_wave < 0 || t->blocks_per_cu < 0 || t->schedule < 0 ||
+  if (t->chunk_cols < 0 || t->lo_chunk_cols < 0 || t->atoms_per_wave < 0 || t->blocks_per_cu < 0 || t->schedule < 0 ||
       t->schedule > 2 || t->order < 0 || t->order > 2 ||
       (t->cols_per_lane != 0 && t->cols_per_lane != 1 && t->cols_per_lane != 2 && t->cols_per_lane != 4))
     return set_err(c, LTB_ERR_ARG, "bad tiling");
@@ -1254,6 +1271,23 @@ int ltb_kpe_autotune(ltb_ctx* c, int precision, int trial_steps, ltb_kpe_tiling*
     }
   }
   cur = win;
+  cur.lo_chunk_cols = 0;
+  // the two-level SCC's low-order recursions: light cones cover less of the
+  // system in narrower chunks, which pays in a short recursion; a whole
+  // order-32 one is timed at the chosen width and at half of it
+  ltb_kpe_tiling r;
+  kpe_resolved_tiling(c, precision, &r);
+  const int half = r.chunk_cols / 2;
+  if (c->light_cone && r.schedule == LTB_SCHED_SWEEP && c->norb > r.chunk_cols && half % 4 == 0 &&
+      half / (64 * r.cols_per_lane) >= 8) {
+    double t_full = 0, t_half = 0;
+    if ((st = kpe_time_recursion(c, precision, 32, r.chunk_cols, 2, &t_full)) ||
+        (st = kpe_time_recursion(c, precision, 32, half, 2, &t_half))) {
+      cur = saved;
+      return st;
+    }
+    if (t_half < t_full) cur.lo_chunk_cols = half;
+  }
   if (best) kpe_resolved_tiling(c, precision, best);
   if (best_gbs) *best_gbs = best_rate / 1e9;
   return LTB_OK;

@@ -125,6 +125,41 @@ def test_two_level_parameter_checks(ctx, c0):
         b.close()
 
 
+def test_two_level_low_order_chunk_width(ctx):
+    """tiling.lo_chunk_cols: the low-order recursions of the two-level SCC at
+    half the chunk width (narrower light cones) follow the run at the full
+    width up to the order of the FP64 dot sums.  4,608 atoms (18,432
+    orbitals), FP64 throughout, 2048-column chunks, low-order chunks of 2048 or
+    1024 columns: the first 12 iterations (1 full-order, 11 at M_lo = 32)
+    agree to 1e-12 e in |dq| and in the charges.  The autotuner picks either
+    width and reports it."""
+    s = L.System.quartz(8, 8, 8, sigma_ang=0.05, seed=1)
+    ctx.load(s)
+    ctx.set_populations(None)
+    lo, hi = ctx.gershgorin()
+    ctx.rescale((hi - lo) / 2, (hi + lo) / 2)
+    for prec in (L.FP64, L.FP32_ACC64):
+        t, _ = ctx.autotune(prec, 3)
+        assert t["lo_chunk_cols"] in (t["chunk_cols"], t["chunk_cols"] // 2), t
+        assert ctx.get_tiling(prec) == t
+    runs = {}
+    try:
+        for lo_cols in (0, 1024):
+            ctx.set_tiling(chunk_cols=2048, cols_per_lane=2, schedule=L.SCHED_SWEEP, blocks_per_cu=2, nontemporal=1,
+                           order=2, lo_chunk_cols=lo_cols)
+            assert ctx.get_tiling(L.FP64)["lo_chunk_cols"] == (lo_cols or 2048)
+            runs[lo_cols] = ctx.scc(512, precision=L.FP64, tol=1e-6, max_iter=12, use_graph=1, M_lo=32, check=False)
+    finally:
+        ctx.set_tiling()
+    a, b = runs[0], runs[1024]
+    print(f"4,608 atoms FP64, low-order chunks 2048 vs 1024 columns: max |d hist| "
+          f"{np.abs(a['hist'] - b['hist']).max():.2e}, max |dq| {np.abs(a['q'] - b['q']).max():.2e} e")
+    for r in (a, b):
+        assert r["status"] == L.LTB_NOT_CONVERGED and r["iters"] == 12 and r["lo_iters"] == 11
+    assert np.abs(a["hist"] - b["hist"]).max() < 1e-12
+    assert np.abs(a["q"] - b["q"]).max() < 1e-12
+
+
 # uneven cells (L, M, N, sigma A, seed), 72 to 162 atoms
 CELLS = [(2, 2, 2, 0.05, 1), (3, 2, 2, 0.10, 7), (2, 3, 2, 0.02, 11), (2, 2, 3, 0.15, 3),
          (3, 3, 2, 0.08, 5)]
