@@ -990,7 +990,7 @@ int ltb_batch_scc_run(ltb_batch* b, const ltb_scc_params* prm, const double* gue
   // falls below switch_tol goes on in plain FP64 iterations
   std::vector<char> low(S, 0);
   std::vector<int> plen(S, 0), stall(S, 0);
-  std::vector<double> best(S, 0.0);
+  std::vector<double> best(S, 0.0), dq_full(S, 0.0);
   const double lo_tol = prm->lo_tol > 0 ? prm->lo_tol : 0.1 * prm->tol;
   std::vector<int> masks(3 * (size_t)S, 0), act_full(S, 0);
   std::vector<double> sc((size_t)S * NSC);
@@ -1077,10 +1077,11 @@ int ltb_batch_scc_run(ltb_batch* b, const ltb_scc_params* prm, const double* gue
           } else {
             stall[s]++;
           }
-          if (p[S_DQ] < lo_tol || stall[s] >= 4 || plen[s] >= 400) low[s] = 0;
+          if (p[S_DQ] < lo_tol || p[S_DQ] < 1e-3 * dq_full[s] || stall[s] >= 4 || plen[s] >= 400) low[s] = 0;
           n32++;
           continue;
         }
+        dq_full[s] = p[S_DQ];
         if (p[S_DQ] < prm->tol && prm->switch_tol <= 0) {
           st_s[s] = LTB_OK;
           active[s] = 0;

@@ -932,8 +932,13 @@ int scc_two_level(ltb_ctx* c, const ltb_scc_params* prm, const double* guess, do
   // or once its smallest |dq| has not fallen by 10% for kStall iterations:
   // in FP32 the corrected map's |dq| stops at the rounding level of the
   // recursion (~1.5e-7 e at 15,552 atoms) while mixing alone shrinks it by
-  // (1 - mix) per iteration
+  // (1 - mix) per iteration.  It also ends below kPhaseRel times the last
+  // full-order |dq|: the next full-order iteration is off by about
+  // rho |dq_full| anyway (rho ~ 0.015 at M_lo = 32, the error of the
+  // corrected map), so resolving the phase's own fixed point more finely
+  // than that buys nothing
   constexpr int kMaxLowPhase = 400, kStall = 4;
+  constexpr double kPhaseRel = 1e-3;
   auto t0 = std::chrono::steady_clock::now();
   int st;
   if (guess)
@@ -954,7 +959,7 @@ int scc_two_level(ltb_ctx* c, const ltb_scc_params* prm, const double* guess, do
   int prec = prm->precision;  // of the full-order iterations
   bool full = true;
   int it = 0, n64 = 0, nlo = 0, ngraph = 0, phase_len = 0, stall = 0;
-  double best = 0;
+  double best = 0, dq_full = 0;
   DevScalars h{};
   const bool saved_timing = c->timing;
   GraphExecs<1> graph;  // the low-order iteration
@@ -1013,6 +1018,7 @@ int scc_two_level(ltb_ctx* c, const ltb_scc_params* prm, const double* guess, do
       if (prec != LTB_FP64 && prm->switch_tol > 0 && h.dq < prm->switch_tol) prec = LTB_FP64;
       full = false;
       phase_len = 0;
+      dq_full = h.dq;
     } else {
       if (phase_len == 1 || h.dq < 0.9 * best) {
         best = h.dq;
@@ -1020,7 +1026,7 @@ int scc_two_level(ltb_ctx* c, const ltb_scc_params* prm, const double* guess, do
       } else {
         stall++;
       }
-      if (h.dq < lo_tol || stall >= kStall || phase_len >= kMaxLowPhase) full = true;
+      if (h.dq < lo_tol || h.dq < kPhaseRel * dq_full || stall >= kStall || phase_len >= kMaxLowPhase) full = true;
     }
   }
   c->timing = saved_timing;

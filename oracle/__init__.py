@@ -288,8 +288,9 @@ def _scc_two_level(o, M, M_lo, guess=None, kT_ev=0.05, K=8192, mix=0.2, tol=1e-6
     """Two-level SCC in FP64 (the control flow of ltb_scc_run with M_lo > 0):
     a full-order iteration evaluates F_M and F_M_lo at q_in, keeps the defect
     d = F_M - F_M_lo and mixes with F_M; low-order iterations mix with
-    F_M_lo + d until their |dq| < lo_tol, or their smallest |dq| has not
-    fallen by 10% for 4 iterations, or max_phase of them; convergence only on
+    F_M_lo + d until their |dq| < lo_tol or < 1e-3 of the last full-order
+    |dq|, or their smallest |dq| has not fallen by 10% for 4 iterations, or
+    max_phase of them; convergence only on
     a full-order iteration with |dq| < tol.  defect_order (test hook) takes
     the defect's low-order term at another order."""
     lo_tol = lo_tol if lo_tol > 0 else 0.1 * tol
@@ -298,7 +299,7 @@ def _scc_two_level(o, M, M_lo, guess=None, kT_ev=0.05, K=8192, mix=0.2, tol=1e-6
     pin = np.array(p0 if guess is None else guess, np.float64)
     d = np.zeros(o.nat)
     full, phase, n_lo, hist, full_hist = True, 0, 0, [], []
-    best, stall = 0.0, 0
+    best, stall, dq_full = 0.0, 0, 0.0
     status = 1
     for it in range(maxit):
         shift = np.repeat(G @ (pin - p0), 4)
@@ -321,13 +322,13 @@ def _scc_two_level(o, M, M_lo, guess=None, kT_ev=0.05, K=8192, mix=0.2, tol=1e-6
             break
         pin = pin + mix * (pout - pin)
         if full:
-            full, phase = False, 0
+            full, phase, dq_full = False, 0, dq
         else:
             if phase == 1 or dq < 0.9 * best:
                 best, stall = dq, 0
             else:
                 stall += 1
-            if dq < lo_tol or stall >= 4 or phase >= max_phase:
+            if dq < lo_tol or dq < 1e-3 * dq_full or stall >= 4 or phase >= max_phase:
                 full = True
     return dict(status=status, pop=pout, q=2 * pout, iters=len(hist), lo_iters=n_lo, hist=np.array(hist),
                 full_hist=np.array(full_hist), mu_ha=r["mu_ha"], eband=r["eband"])
