@@ -751,6 +751,9 @@ def main():
                     help="only run 1 GiB device-to-device copies (counter calibration)")
     ap.add_argument("--no-tts", action="store_true",
                     help="default run: leave out the time-to-solution leg")
+    ap.add_argument("--no-light-cone", action="store_true",
+                    help="default run: every recursion step visits every atom (counter runs of the full-width "
+                         "step)")
     ap.add_argument("--time-budget", type=float, default=550.0,
                     help="default run: seconds since start within which the time-to-solution leg must end")
     args = ap.parse_args()
@@ -796,6 +799,8 @@ def main():
     log(f"structure {args.cells}: {s.nat} atoms, {s.nnzb} blocks, {time.perf_counter() - t0:.2f} s")
     ctx = L.Context(0)
     ctx.load(s)
+    if args.no_light_cone:
+        ctx.set_light_cone(False)
     tiling = None
     if args.tiling:
         vals = [int(x) for x in args.tiling.split(",")]
