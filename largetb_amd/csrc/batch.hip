@@ -1077,7 +1077,9 @@ int ltb_batch_scc_run(ltb_batch* b, const ltb_scc_params* prm, const double* gue
           } else {
             stall[s]++;
           }
-          if (p[S_DQ] < lo_tol || p[S_DQ] < 1e-3 * dq_full[s] || stall[s] >= 4 || plen[s] >= 400) low[s] = 0;
+          if (p[S_DQ] < lo_tol || p[S_DQ] < kTwoLevelPhaseRel * dq_full[s] || stall[s] >= kTwoLevelStall ||
+              plen[s] >= kTwoLevelMaxPhase)
+            low[s] = 0;
           n32++;
           continue;
         }

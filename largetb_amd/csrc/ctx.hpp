@@ -96,6 +96,19 @@ struct ltb_ctx {
 
 int set_err(ltb_ctx* c, int code, const std::string& msg);
 
+// End of a low-order phase of the two-level SCC (ltb_scc_run and the batch
+// alike): after kTwoLevelMaxPhase iterations even if its |dq| is still above
+// lo_tol (linear mixing 0.2 needs ~70 from a neutral start); once its
+// smallest |dq| has not fallen by 10% for kTwoLevelStall iterations (in FP32
+// the corrected map's |dq| stops at the rounding level of the recursion,
+// ~1.5e-7 e at 15,552 atoms, while mixing alone shrinks it by 1 - mix per
+// iteration); or below kTwoLevelPhaseRel times the last full-order |dq|: the
+// next full-order iteration is off by about rho |dq_full| anyway (rho ~ 0.015
+// at M_lo = 32, the error of the corrected map), so resolving the phase's own
+// fixed point more finely than that buys nothing.
+constexpr int kTwoLevelMaxPhase = 400, kTwoLevelStall = 4;
+constexpr double kTwoLevelPhaseRel = 1e-3;
+
 // instantiated graphs of one SCC run, destroyed on every exit path
 template <int N>
 struct GraphExecs {

@@ -927,18 +927,6 @@ int scc_low_iteration_async(ltb_ctx* c, const ltb_scc_params* prm, const ltb_fer
 // quantities.  Convergence is declared on full-order iterations only.
 int scc_two_level(ltb_ctx* c, const ltb_scc_params* prm, const double* guess, double* p_out, double* hist,
                   ltb_scc_result* res) {
-  // a low-order phase ends after this many iterations even if its |dq| is
-  // still above lo_tol (linear mixing 0.2 needs ~70 from a neutral start),
-  // or once its smallest |dq| has not fallen by 10% for kStall iterations:
-  // in FP32 the corrected map's |dq| stops at the rounding level of the
-  // recursion (~1.5e-7 e at 15,552 atoms) while mixing alone shrinks it by
-  // (1 - mix) per iteration.  It also ends below kPhaseRel times the last
-  // full-order |dq|: the next full-order iteration is off by about
-  // rho |dq_full| anyway (rho ~ 0.015 at M_lo = 32, the error of the
-  // corrected map), so resolving the phase's own fixed point more finely
-  // than that buys nothing
-  constexpr int kMaxLowPhase = 400, kStall = 4;
-  constexpr double kPhaseRel = 1e-3;
   auto t0 = std::chrono::steady_clock::now();
   int st;
   if (guess)
@@ -1026,7 +1014,9 @@ int scc_two_level(ltb_ctx* c, const ltb_scc_params* prm, const double* guess, do
       } else {
         stall++;
       }
-      if (h.dq < lo_tol || h.dq < kPhaseRel * dq_full || stall >= kStall || phase_len >= kMaxLowPhase) full = true;
+      if (h.dq < lo_tol || h.dq < kTwoLevelPhaseRel * dq_full || stall >= kTwoLevelStall ||
+          phase_len >= kTwoLevelMaxPhase)
+        full = true;
     }
   }
   c->timing = saved_timing;
