@@ -1243,8 +1243,8 @@ int ltb_kpe_autotune(ltb_ctx* c, int precision, int trial_steps, ltb_kpe_tiling*
   for (int ch : chunks)
     for (int v : vs)
       for (int nt = 0; nt <= 1; nt++) {
-        cand.push_back({ch, v, LTB_SCHED_BLOCKED, 0, 0, nt, saved.order});
-        for (int bpc : {2, 4, 8}) cand.push_back({ch, v, LTB_SCHED_SWEEP, bpc, 0, nt, saved.order});
+        cand.push_back({ch, v, LTB_SCHED_BLOCKED, 0, 0, nt, saved.order, 0});
+        for (int bpc : {2, 4, 8}) cand.push_back({ch, v, LTB_SCHED_SWEEP, bpc, 0, nt, saved.order, 0});
       }
   double best_rate = -1;
   ltb_kpe_tiling win = saved;
