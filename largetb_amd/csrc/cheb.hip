@@ -40,6 +40,7 @@
 
 #include "cheb_device.hpp"
 #include "ctx.hpp"
+#include "system.hpp"
 
 namespace {
 
@@ -362,42 +363,6 @@ void free_cones_impl(ltb_ctx* c) {
   c->cones.clear();
 }
 
-// runs of `in` (cyclic), merged across the smallest gaps down to kMaxRanges
-void cone_ranges(const std::vector<char>& in, int nat, std::vector<std::pair<int, int>>& out) {
-  out.clear();
-  for (int a = 0; a < nat;) {
-    if (!in[a]) {
-      a++;
-      continue;
-    }
-    int b = a;
-    while (b < nat && in[b]) b++;
-    out.push_back({a, b - a});
-    a = b;
-  }
-  if (out.size() > 1 && out.front().first == 0 && out.back().first + out.back().second == nat) {
-    out.back().second += out.front().second;  // a run across the end of the atom order
-    out.erase(out.begin());
-  }
-  for (int gap = 8; (int)out.size() > kMaxRanges; gap *= 2) {
-    std::vector<std::pair<int, int>> m;
-    for (const auto& r : out) {
-      if (!m.empty()) {
-        const int end = m.back().first + m.back().second;  // may exceed nat only for the last run
-        if (r.first - end <= gap) {
-          m.back().second = r.first + r.second - m.back().first;
-          continue;
-        }
-      }
-      m.push_back(r);
-    }
-    out.swap(m);
-  }
-  int tot = 0;
-  for (const auto& r : out) tot += r.second;
-  if (tot >= nat) out.assign(1, {0, nat});
-}
-
 int build_cones(ltb_ctx* c, int ld) {
   if (c->cones.count(ld)) return LTB_OK;
   const int nat = c->nat;
@@ -408,38 +373,8 @@ int build_cones(ltb_ctx* c, int ld) {
     c->h_rowptr.swap(rp);
     c->h_col.swap(cl);
   }
-  const int apc = ld / 4;  // atoms per chunk
   ltb_ctx::ConeTable t;
-  std::vector<char> in(nat);
-  std::vector<int> frontier, next;
-  std::vector<std::pair<int, int>> rs;
-  for (int a0 = 0; a0 < nat; a0 += apc) {
-    const int a1 = std::min(a0 + apc, nat);
-    std::fill(in.begin(), in.end(), 0);
-    frontier.clear();
-    for (int a = a0; a < a1; a++) {
-      in[a] = 1;
-      frontier.push_back(a);
-    }
-    std::vector<std::pair<int, int>> idx;
-    for (;;) {
-      cone_ranges(in, nat, rs);
-      idx.push_back({(int)t.ranges.size(), (int)rs.size()});
-      t.ranges.insert(t.ranges.end(), rs.begin(), rs.end());
-      if (frontier.empty() || (rs.size() == 1 && rs[0].second >= nat)) break;
-      next.clear();
-      for (int a : frontier)
-        for (int k = c->h_rowptr[a]; k < c->h_rowptr[a + 1]; k++) {
-          const int b = c->h_col[k];
-          if (!in[b]) {
-            in[b] = 1;
-            next.push_back(b);
-          }
-        }
-      frontier.swap(next);
-    }
-    t.idx.push_back(std::move(idx));
-  }
+  ltb::light_cones(nat, c->h_rowptr.data(), c->h_col.data(), ld / 4, kMaxRanges, t.idx, t.ranges);
   std::vector<int> flat;
   for (const auto& r : t.ranges) {
     flat.push_back(r.first);

@@ -5,8 +5,11 @@
 // both lattices, the cell list and the image-by-image scan, a cluster
 // without a cell, the error paths (atoms on top of each other, too many
 // neighbours), the descriptor pair search, the visiting order, and the MATLAB
-// files (round trip, truncated and malformed input).  Prints "host_check ok"
-// and exits 0 when every check holds.
+// files (round trip, truncated and malformed input), and the light-cone
+// tables of the Chebyshev sweep (against an independent distance search on
+// quartz, ring and random graphs).  Prints "host_check ok" and exits 0 when
+// every check holds.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -148,6 +151,114 @@ void matlab_files(const std::string& dir) {
   check(ltb_read_matlab_vec((dir + "/missing.m").c_str(), &n, nullptr) != LTB_OK, "missing file");
 }
 
+// every atom within h hops of the chunk (an independent breadth-first
+// distance search) lies in one of the chunk's ranges for hop h
+void check_cones(int nat, const std::vector<int>& rp, const std::vector<int>& col, int apc, int max_ranges,
+                 const std::string& what) {
+  std::vector<std::vector<std::pair<int, int>>> idx;
+  std::vector<std::pair<int, int>> ranges;
+  ltb::light_cones(nat, rp.data(), col.data(), apc, max_ranges, idx, ranges);
+  const int nchunk = (nat + apc - 1) / apc;
+  check((int)idx.size() == nchunk, what + ": one cone list per chunk");
+  for (int k = 0; k < nchunk && k < (int)idx.size(); k++) {
+    std::vector<int> dist(nat, -1), queue;
+    for (int a = k * apc; a < std::min((k + 1) * apc, nat); a++) {
+      dist[a] = 0;
+      queue.push_back(a);
+    }
+    for (size_t q = 0; q < queue.size(); q++)
+      for (int e = rp[queue[q]]; e < rp[queue[q] + 1]; e++)
+        if (dist[col[e]] < 0) {
+          dist[col[e]] = dist[queue[q]] + 1;
+          queue.push_back(col[e]);
+        }
+    int hmax = 0;
+    for (int d : dist) hmax = std::max(hmax, d);
+    bool ok = !idx[k].empty();
+    for (int h = 0; ok && h <= hmax + 1; h++) {
+      const auto& e = idx[k][std::min<size_t>(h, idx[k].size() - 1)];
+      ok = e.second >= 1 && e.second <= std::max(1, max_ranges) && e.first + e.second <= (int)ranges.size();
+      std::vector<char> cover(nat, 0);
+      for (int r = e.first; ok && r < e.first + e.second; r++) {
+        const int lo = ranges[r].first, len = ranges[r].second;
+        ok = lo >= 0 && lo < nat && len >= 1 && len <= nat;
+        for (int i = 0; ok && i < len; i++) cover[(lo + i) % nat] = 1;
+      }
+      for (int a = 0; ok && a < nat; a++)
+        if (dist[a] >= 0 && dist[a] <= h && !cover[a]) ok = false;
+    }
+    check(ok, what + ": chunk " + std::to_string(k) + " cones cover every atom within reach");
+  }
+}
+
+// a ring a - 1, a, a + 1 (mod n) over [first, first + n)
+void ring(int first, int n, std::vector<int>& rp, std::vector<int>& col) {
+  for (int a = 0; a < n; a++) {
+    col.push_back(first + a);
+    col.push_back(first + (a + 1) % n);
+    col.push_back(first + (a + n - 1) % n);
+    rp.push_back((int)col.size());
+  }
+}
+
+void light_cone_paths() {
+  ltb_system s;
+  ltb::Status st = ltb::build_quartz(3, 3, 3, LTB_LATTICE_CONFIG, 0.05, 1, s);
+  check(st.code == 0, "quartz 3x3x3 for the cones");
+  for (int apc : {1, 16, 64, 100, 243})
+    for (int mr : {1, 2, 32})
+      check_cones(s.natoms, s.h_rowptr, s.h_col, apc, mr,
+                  "quartz cones apc " + std::to_string(apc) + " ranges " + std::to_string(mr));
+  // one ring: the exact cone of chunk 0 at hop h is one cyclic run, its ranges say so
+  std::vector<int> rp{0}, col;
+  ring(0, 100, rp, col);
+  std::vector<std::vector<std::pair<int, int>>> idx;
+  std::vector<std::pair<int, int>> ranges;
+  ltb::light_cones(100, rp.data(), col.data(), 10, 32, idx, ranges);
+  bool exact = idx.size() == 10 && idx[0].size() == 46;
+  for (int h = 1; exact && h < 45; h++) {
+    const auto& e = idx[0][h];
+    exact = e.second == 1 && ranges[e.first].first == 100 - h && ranges[e.first].second == 10 + 2 * h;
+  }
+  check(exact && ranges[idx[0].back().first] == std::make_pair(0, 100), "ring cones wrap as one range");
+  for (int mr : {1, 4, 32}) check_cones(100, rp, col, 10, mr, "ring " + std::to_string(mr));
+  // two rings: a cone never leaves its own ring, and its list stops when it stops growing
+  std::vector<int> rp2{0}, col2;
+  ring(0, 50, rp2, col2);
+  ring(50, 50, rp2, col2);
+  ltb::light_cones(100, rp2.data(), col2.data(), 25, 32, idx, ranges);
+  const auto& last = idx[0].back();
+  check(last.second == 1 && ranges[last.first] == std::make_pair(0, 50), "two rings: the cone stays in its ring");
+  for (int mr : {1, 3, 32}) check_cones(100, rp2, col2, 25, mr, "two rings " + std::to_string(mr));
+  // random symmetric graphs with scattered neighbours (many runs to merge)
+  uint64_t x = 12345;
+  auto rnd = [&x](int n) {
+    x = x * 6364136223846793005ULL + 1442695040888963407ULL;
+    return (int)((x >> 33) % (uint64_t)n);
+  };
+  for (int n : {37, 200}) {
+    std::vector<std::vector<int>> adj(n);
+    for (int a = 0; a < n; a++) {
+      adj[a].push_back(a);
+      for (int t = 0; t < 2; t++) {
+        const int b = rnd(n);
+        if (b != a) {
+          adj[a].push_back(b);
+          adj[b].push_back(a);
+        }
+      }
+    }
+    std::vector<int> rpr{0}, colr;
+    for (auto& v : adj) {
+      colr.insert(colr.end(), v.begin(), v.end());
+      rpr.push_back((int)colr.size());
+    }
+    for (int apc : {1, 7, 64})
+      for (int mr : {1, 2, 5, 32})
+        check_cones(n, rpr, colr, apc, mr, "random graph " + std::to_string(n));
+  }
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -155,6 +266,7 @@ int main(int argc, char** argv) {
   quartz_paths();
   cluster_and_errors();
   matlab_files(dir);
+  light_cone_paths();
   std::vector<double> g(1000);
   ltb::gaussian_fill(3, g.size(), g.data());
   double m = 0;

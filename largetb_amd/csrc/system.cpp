@@ -520,4 +520,75 @@ void gaussian_fill(uint64_t seed, size_t n, double* out) {
   for (size_t k = 0; k < n; k++) out[k] = g.normal();
 }
 
+void cone_ranges(const std::vector<char>& in, int nat, int max_ranges, std::vector<std::pair<int, int>>& out) {
+  out.clear();
+  for (int a = 0; a < nat;) {
+    if (!in[a]) {
+      a++;
+      continue;
+    }
+    int b = a;
+    while (b < nat && in[b]) b++;
+    out.push_back({a, b - a});
+    a = b;
+  }
+  if (out.size() > 1 && out.front().first == 0 && out.back().first + out.back().second == nat) {
+    out.back().second += out.front().second;  // a run across the end of the atom order
+    out.erase(out.begin());
+  }
+  for (long long gap = 8; (int)out.size() > std::max(1, max_ranges); gap *= 2) {
+    std::vector<std::pair<int, int>> m;
+    for (const auto& r : out) {
+      if (!m.empty()) {
+        const int end = m.back().first + m.back().second;  // may pass nat only for the last run
+        if (r.first - end <= gap) {
+          m.back().second = r.first + r.second - m.back().first;
+          continue;
+        }
+      }
+      m.push_back(r);
+    }
+    out.swap(m);
+  }
+  long long tot = 0;
+  for (const auto& r : out) tot += r.second;
+  if (tot >= nat) out.assign(1, {0, nat});
+}
+
+void light_cones(int nat, const int* rowptr, const int* col, int apc, int max_ranges,
+                 std::vector<std::vector<std::pair<int, int>>>& idx, std::vector<std::pair<int, int>>& ranges) {
+  idx.clear();
+  ranges.clear();
+  std::vector<char> in(nat);
+  std::vector<int> frontier, next;
+  std::vector<std::pair<int, int>> rs;
+  for (int a0 = 0; a0 < nat; a0 += apc) {
+    const int a1 = std::min(a0 + apc, nat);
+    std::fill(in.begin(), in.end(), 0);
+    frontier.clear();
+    for (int a = a0; a < a1; a++) {
+      in[a] = 1;
+      frontier.push_back(a);
+    }
+    std::vector<std::pair<int, int>> hops;
+    for (;;) {
+      cone_ranges(in, nat, max_ranges, rs);
+      hops.push_back({(int)ranges.size(), (int)rs.size()});
+      ranges.insert(ranges.end(), rs.begin(), rs.end());
+      if (frontier.empty() || (rs.size() == 1 && rs[0].second >= nat)) break;
+      next.clear();
+      for (int a : frontier)
+        for (int k = rowptr[a]; k < rowptr[a + 1]; k++) {
+          const int b = col[k];
+          if (!in[b]) {
+            in[b] = 1;
+            next.push_back(b);
+          }
+        }
+      frontier.swap(next);
+    }
+    idx.push_back(std::move(hops));
+  }
+}
+
 }  // namespace ltb

@@ -3,6 +3,7 @@
 #pragma once
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 struct ltb_system {
@@ -51,4 +52,19 @@ Status locality_order(const ltb_system& s, std::vector<int>& order);
 // n standard normal numbers of the SplitMix64 -> Box-Muller stream of `seed`
 // (the generator of the structure displacements)
 void gaussian_fill(uint64_t seed, size_t n, double* out);
+
+// The atoms marked in `in` (nat entries) as at most max_ranges cyclic ranges
+// {lo, len}: its runs of consecutive atoms (a run across the end of the order
+// wraps: lo + len > nat), merged across the smallest gaps until few enough;
+// {0, nat} when they hold every atom.  Every marked atom lies in a range.
+void cone_ranges(const std::vector<char>& in, int nat, int max_ranges, std::vector<std::pair<int, int>>& out);
+
+// Light cones over the block graph of H (rowptr / col, nat block rows): for
+// chunk k of the atoms (k*apc .. min((k+1)*apc, nat) - 1) and hop count
+// h = 0, 1, ..., the atoms within h hops of the chunk's atoms, as
+// cone_ranges gives them.  idx[k][h] = {first, count} into `ranges`; a chunk's
+// list stops once its cone holds every atom or stops growing, and its last
+// entry stands for every larger h.
+void light_cones(int nat, const int* rowptr, const int* col, int apc, int max_ranges,
+                 std::vector<std::vector<std::pair<int, int>>>& idx, std::vector<std::pair<int, int>>& ranges);
 }  // namespace ltb
