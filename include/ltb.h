@@ -413,7 +413,9 @@ int ltb_get_kpe_tiling(ltb_ctx* ctx, int precision, ltb_kpe_tiling* out);
  * precision's tiling is left as it was) and report it with its algorithmic
  * GB/s.  With light cones on and a sweep over several chunks it then times a
  * whole order-32 recursion at the chosen chunk width and at half of it and
- * keeps the faster as lo_chunk_cols. */
+ * keeps the faster as lo_chunk_cols.  The trials overwrite the recursion
+ * buffers and the stored moments: run ltb_kpe_moments again before
+ * ltb_density. */
 int ltb_kpe_autotune(ltb_ctx* ctx, int precision, int trial_steps, ltb_kpe_tiling* best,
                      double* best_gbs);
 
