@@ -945,6 +945,7 @@ int ltb_batch_scc_run(ltb_batch* b, const ltb_scc_params* prm, const double* gue
       if (*p) (void)hipFree(*p);
       *p = nullptr;
     }
+    b->M = b->K = 0;  // no set looks valid until every allocation below succeeded
     LTB_HIP(c, hipMalloc(&b->d_mom, sizeof(double) * (size_t)(M + 1) * b->nat));
     LTB_HIP(c, hipMalloc(&b->d_tot, sizeof(double) * (size_t)(M + 1) * S));
     LTB_HIP(c, hipMalloc(&b->d_W, sizeof(double) * (size_t)K * S));
@@ -963,6 +964,7 @@ int ltb_batch_scc_run(ltb_batch* b, const ltb_scc_params* prm, const double* gue
       if (*p) (void)hipFree(*p);
       *p = nullptr;
     }
+    b->M_lo = 0;
     LTB_HIP(c, hipMalloc(&b->d_tot_lo, sizeof(double) * (size_t)(L + 1) * S));
     LTB_HIP(c, hipMalloc(&b->d_g_lo, sizeof(double) * L));
     LTB_HIP(c, hipMalloc(&b->d_w_lo, sizeof(double) * (size_t)L * S));
@@ -971,11 +973,11 @@ int ltb_batch_scc_run(ltb_batch* b, const ltb_scc_params* prm, const double* gue
     LTB_HIP(c, hipMemcpy(b->d_g_lo, g.data(), sizeof(double) * L, hipMemcpyHostToDevice));
     b->M_lo = L;
   }
-  if (two && !b->d_masks) {
+  if (two && (!b->d_masks || !b->d_plo || !b->d_defect)) {
     int st1;
-    if ((st1 = bmalloc(b, &b->d_masks, sizeof(int) * 3 * (size_t)S)) ||
-        (st1 = bmalloc(b, &b->d_plo, sizeof(double) * b->nat)) ||
-        (st1 = bmalloc(b, &b->d_defect, sizeof(double) * b->nat)))
+    if ((!b->d_masks && (st1 = bmalloc(b, &b->d_masks, sizeof(int) * 3 * (size_t)S))) ||
+        (!b->d_plo && (st1 = bmalloc(b, &b->d_plo, sizeof(double) * b->nat))) ||
+        (!b->d_defect && (st1 = bmalloc(b, &b->d_defect, sizeof(double) * b->nat))))
       return st1;
   }
   if (guess)
