@@ -191,6 +191,26 @@ __global__ __launch_bounds__(256) void k_cheb_sweep(const int* __restrict__ rowp
 // blocked schedule)
 __global__ __launch_bounds__(256) void k_cheb_reduce(FoldArgs f) { fold_columns(f, blockIdx.x); }
 
+// Zero the 4 rows (ld elements each, contiguous) of every atom of the cyclic
+// ranges {lo, len} (count = the ranges' total length), or of atoms 0 .. count-1
+// without ranges.  Workgroup i takes the i-th atom of the concatenated ranges.
+template <typename T>
+__global__ __launch_bounds__(256) void k_zero_atoms(T* __restrict__ buf, const int* __restrict__ ranges, int nranges,
+                                                    int nat, int ld, int count) {
+  const int n16 = (int)(4 * (size_t)ld * sizeof(T) / 16);
+  for (int i = blockIdx.x; i < count; i += gridDim.x) {
+    int A = i;
+    if (ranges) {
+      int base = 0, r = 0;
+      while (r < nranges - 1 && i >= base + ranges[2 * r + 1]) base += ranges[2 * r + 1], r++;
+      A = ranges[2 * r] + (i - base);
+      A -= A >= nat ? nat : 0;
+    }
+    float4* row = reinterpret_cast<float4*>(buf + (size_t)4 * A * ld);
+    for (int k = threadIdx.x; k < n16; k += 256) row[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
 __global__ __launch_bounds__(256) void k_mom_total(const double* __restrict__ mom, int nat,
                                                    double* __restrict__ tot) {
   const int m = blockIdx.x;
@@ -346,11 +366,12 @@ void launch_step(const ltb_ctx* c, int mode, const T* hv, const T* xcur, T* xnex
 
 // Light cones of the unit-vector chunks.  t_{n+1} = T_{n+1}(H~) e_j is zero
 // on every atom more than n + 1 hops (in H~'s block graph) from atom j/4, so
-// step n >= 2 of the chunk j0 .. j0 + ld - 1 only has to visit the atoms
-// within n + 1 hops of its own.  Steps 0 and 1 write every row.  A row a
-// later step leaves alone is then outside that step's cone and so outside
-// every earlier one: each value the buffers ever held there is an exact zero
-// (a product or sum of zeros), which is what its readers need.  Visiting a
+// step n of the chunk j0 .. j0 + ld - 1 only has to visit the atoms within
+// n + 1 hops of its own.  Both buffers start the chunk all zero (see
+// run_moments).  A row a step leaves alone is then outside that step's cone
+// and so outside every earlier one: each value the buffers ever held there is
+// an exact zero (a product or sum of zeros), which is what its readers need,
+// and a chunk leaves non-zeros only inside its last cone.  Visiting a
 // superset of the cone is therefore always safe; per chunk and hop count h
 // the table holds the cone as at most kMaxRanges cyclic atom ranges {lo, len}
 // (runs of consecutive atoms, the closest runs merged), or {0, nat} once it
@@ -435,6 +456,18 @@ int run_moments(ltb_ctx* c, int M) {
   if (cones && (st = build_cones(c, p.ld))) return st;
   int launches = 0;
   double bytes = 0;
+  // With light cones every step of a chunk visits only its cone, so the
+  // buffers must hold zeros wherever a step reads rows it has not written:
+  // both are cleared before the first chunk, and after a chunk only its last
+  // cone (the one place it left anything but zeros) is cleared again.
+  bool clear = true;
+  const int* dirty = nullptr;  // device ranges of the previous chunk's last cone
+  int ndirty = 0, dirty_atoms = 0, dirty_ldc = 0;
+  auto zero_atoms = [&](T* buf, const int* d_ranges, int nranges, int count, int ldc) {
+    if (count > 0)
+      hipLaunchKernelGGL(k_zero_atoms<T>, dim3(std::min(count, 65535)), dim3(256), 0, c->stream, buf, d_ranges,
+                         nranges, c->nat, ldc, count);
+  };
   for (int j0 = 0; j0 < c->norb; j0 += p.ld) {
     const int nvalid = std::min(p.ld, c->norb - j0);
     const int ldc = (nvalid + p.W - 1) / p.W * p.W;
@@ -460,16 +493,38 @@ int run_moments(ltb_ctx* c, int M) {
                       ldc, nvalid, j0, n, M, c->d_mu1, c->d_mom, c->nat,
                       c->keep_orbital ? c->d_momorb : nullptr, c->norb};
     };
+    const bool chunk_cones = cones && sweep;
+    if (chunk_cones) {
+      if (clear || ldc != dirty_ldc) {  // a narrower last chunk lays its rows out at its own width
+        zero_atoms(A, nullptr, 0, c->nat, ldc);
+        zero_atoms(B, nullptr, 0, c->nat, ldc);
+      } else if (ndirty > 0) {
+        zero_atoms(A, dirty, ndirty, dirty_atoms, ldc);
+        zero_atoms(B, dirty, ndirty, dirty_atoms, ldc);
+      }
+      LTB_HIP(c, hipGetLastError());
+      const auto& t = c->cones[p.ld];
+      const auto& ci = t.idx[j0 / p.ld];
+      const auto& e = ci[std::min<size_t>(nsteps, ci.size() - 1)];  // holds t_{nsteps-1} and t_{nsteps}
+      dirty = t.d_ranges + 2 * (size_t)e.first;
+      ndirty = e.second;
+      dirty_atoms = 0;
+      for (int q = 0; q < e.second; q++) dirty_atoms += t.ranges[e.first + q].second;
+      dirty_ldc = ldc;
+      clear = false;
+    } else {
+      clear = true;  // a chunk without cones writes every row at its own width
+    }
     for (int n = 0; n < nsteps; n++) {
       const int mode = n == 0 ? 0 : (n == 1 ? 1 : 2);
       T* cur = (n & 1) ? B : A;
       T* nxt = (n == 0) ? B : ((n & 1) ? A : B);
       // under the sweep schedule step n folds step n - 1's partials itself
       const FoldArgs prev = sweep && n > 0 ? fold_args(n - 1) : FoldArgs{};
-      // steps 0 and 1 write every row; later sweep steps visit the cone of t_{n+1}
+      // step n visits the cone of t_{n+1} (every row without light cones)
       AtomRange r;
       const std::pair<int, int>* rr = nullptr;
-      if (cones && sweep && n >= 2) {
+      if (chunk_cones) {
         const auto& t = c->cones[p.ld];
         const auto& ci = t.idx[j0 / p.ld];
         const auto& e = ci[std::min<size_t>(n + 1, ci.size() - 1)];
